@@ -1,0 +1,131 @@
+/*
+ * flct_hip.h -- C ABI of the MI355X (gfx950) FLCT tracker and Fourier warp.
+ *
+ * Arrays are C-space, row-major: element (i, j) of an nx x ny image is at
+ * i*ny + j, where nx is the IDL y size and ny the IDL x size (the reference
+ * runs with transp = 1, flct.c:291; the vcimage header is read swapped,
+ * flct.c:1192-1196).  No torch or HIP types appear in these signatures.
+ *
+ * Every entry point returns a status code (FLCT_HIP_OK = 0).  Each code that
+ * corresponds to an exit(1) site of the reference CLI is named after it.
+ * A context is bound to one GPU and one HIP stream; calls on one context are
+ * serialised by the caller (the reference is single-threaded).
+ */
+#ifndef FLCT_HIP_H
+#define FLCT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FLCT_HIP_OK 0
+#define FLCT_HIP_E_ARG 1           /* bad argument (null pointer, size <= 0) */
+#define FLCT_HIP_E_SKIP_NONPOS 2   /* flct.c:459-464  "skip =%d is zero or negative" */
+#define FLCT_HIP_E_PQ_RANGE 3      /* flct.c:466-472  "abs(p) or abs(q) >= skip" */
+#define FLCT_HIP_E_SKIP_TOO_BIG 4  /* flct.c:559-563  "skip = %d is too big" */
+#define FLCT_HIP_E_BOX 5           /* flct.c:575-579  "illegal box size" */
+#define FLCT_HIP_E_UNSUPPORTED 6   /* a request this build does not handle */
+#define FLCT_HIP_E_NODEVICE 7      /* no usable gfx950 device */
+#define FLCT_HIP_E_ALLOC 8         /* device allocation failed */
+#define FLCT_HIP_E_HIP 9           /* a HIP runtime call failed */
+#define FLCT_HIP_E_SHAPE 10        /* warp.c:182-188 "bad shift arrays" */
+
+/* arithmetic forms of the tracker */
+#define FLCT_HIP_FP64 0        /* plain form: FP64 throughout (drop-in) */
+#define FLCT_HIP_FP32 1        /* FP32 transforms, FP64 means, fit and outputs */
+#define FLCT_HIP_SPLIT_BF16 2  /* DFTs as split-BF16 MFMA contractions, FP32 accumulate */
+
+typedef struct flct_hip_ctx flct_hip_ctx;
+
+/* Options of one flct run: the reference command line
+ * "flct ifile ofile deltat deltas sigma [-t thr[a]] [-k kr] [-s N[pP][qQ][i]] [-b] [-q]"
+ * (flct.c:304, flct.c:370-487). */
+typedef struct flct_hip_opts {
+  double deltat;     /* time between images */
+  double deltas;     /* length unit of one pixel */
+  double sigma;      /* Gaussian window width in pixels; must be > 0 */
+  double thresh;     /* -t value (0: off) */
+  int thresh_abs;    /* -t value carried an 'a' suffix: absolute units */
+  int skip;          /* -s N (0: off) */
+  int poff;          /* -s ...pP: offset along IDL x (C-space j) */
+  int qoff;          /* -s ...qQ: offset along IDL y (C-space i) */
+  int interp;        /* -s ...i: fill skipped pixels by cubic convolution */
+  int filter;        /* -k given */
+  double kr;         /* -k value */
+  int bias;          /* -b: window-offset bias correction (not in the reference) */
+  int precision;     /* FLCT_HIP_FP64, FLCT_HIP_FP32, FLCT_HIP_SPLIT_BF16 */
+  int quiet;         /* -q: no progress messages */
+} flct_hip_opts;
+
+/* Fill *opts with the reference defaults (flct.c:370-375): thresh 0, no
+ * skip, no filter, FP64. */
+void flct_hip_opts_default(flct_hip_opts *opts);
+
+/* Human-readable text for a status code. */
+const char *flct_hip_strerror(int status);
+
+/* Create a context on HIP device `device`.  Allocates nothing large. */
+int flct_hip_create(flct_hip_ctx **ctx, int device);
+int flct_hip_destroy(flct_hip_ctx *ctx);
+
+/*
+ * flct_hip_track replaces the per-pixel body of flct's main program,
+ * flct.c:567-1133: box size, relative threshold, the i,j loop with its
+ * threshold and skip rules, the windowed FFT cross-correlation of
+ * cross_cor (flct.c:1325-1649, hires off), the skip fill by interpcc2d
+ * (flct.c:958-1120, 2226-2463) and the final xmiss reset.
+ * f1, f2: host arrays (nx*ny doubles, C-space).  vx, vy, vm: caller-allocated
+ * host arrays of nx*ny doubles.  sigma = 0 (whole-image shift) returns
+ * FLCT_HIP_E_UNSUPPORTED.
+ */
+int flct_hip_track(flct_hip_ctx *ctx, const double *f1, const double *f2, int nx, int ny,
+                   const flct_hip_opts *opts, double *vx, double *vy, double *vm);
+
+/* Same as flct_hip_track with every array already in device memory (HIP
+ * device pointers); the call is ordered on the context's stream and returns
+ * after the outputs are complete. */
+int flct_hip_track_device(flct_hip_ctx *ctx, const double *d_f1, const double *d_f2, int nx,
+                          int ny, const flct_hip_opts *opts, double *d_vx, double *d_vy,
+                          double *d_vm);
+
+/* Number of pixels whose cross-correlation the last track call computed
+ * (not thresholded, not skipped), and the device time of its tracking
+ * kernel in milliseconds (HIP events on the context's stream). */
+int flct_hip_last_stats(flct_hip_ctx *ctx, long long *tracked, double *kernel_ms,
+                        double *total_ms);
+
+/*
+ * flct_hip_warp replaces warp_frac2d (warp.c:883-1011): out(i,j) is the
+ * image's Fourier series evaluated at (i - shiftx, j - shifty), where with
+ * transp != 0 shiftx = dely[i*ny+j], shifty = delx[i*ny+j] (warp.c:948-957).
+ * Host arrays of nx*ny doubles; out caller-allocated.
+ */
+int flct_hip_warp(flct_hip_ctx *ctx, const double *arr, const double *delx,
+                  const double *dely, double *outarr, int nx, int ny, int transp, int quiet);
+
+/* Device-pointer form of flct_hip_warp. */
+int flct_hip_warp_device(flct_hip_ctx *ctx, const double *d_arr, const double *d_delx,
+                         const double *d_dely, double *d_out, int nx, int ny, int transp);
+
+/*
+ * flct_hip_shift replaces shift_frac2d (warp.c:757-881): a uniform
+ * sub-pixel shift by a Fourier phase ramp.  Host arrays.
+ */
+int flct_hip_shift(flct_hip_ctx *ctx, const double *arr, double delx, double dely,
+                   double *outarr, int nx, int ny, int transp, int quiet);
+
+/* Device memory helpers for callers of the *_device entry points (no
+ * counterpart in the reference, which has no device).  Allocations belong to
+ * the context's device; copies are ordered on the context's stream and
+ * complete before return. */
+int flct_hip_malloc(flct_hip_ctx *ctx, void **dptr, long long bytes);
+int flct_hip_free(flct_hip_ctx *ctx, void *dptr);
+int flct_hip_upload(flct_hip_ctx *ctx, void *dptr, const void *src, long long bytes);
+int flct_hip_download(flct_hip_ctx *ctx, void *dst, const void *dptr, long long bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLCT_HIP_H */

@@ -1,0 +1,225 @@
+"""Python interface of the MI355X FLCT tracker and Fourier shifter.
+
+Mirrors the reference programs' operator interface:
+
+* ``flct(f1, f2, deltat, deltas, sigma, ...)`` is ``flct ifile ofile deltat
+  deltas sigma [-t thr[a]] [-k kr] [-s N[pP][qQ][i]] [-b]`` (flct.c:304,
+  flct.c:370-487) on in-memory C-space arrays, returning ``(vx, vy, vm)``;
+* ``warp(arr, delx, dely)`` is ``warp_frac2d`` (warp.c:883-1011);
+* ``shift_frac2d(arr, delx, dely)`` is ``shift_frac2d`` (warp.c:757-881).
+
+Option errors raise :class:`FlctHipError` whose ``status`` names the
+reference's exit(1) site (see include/flct_hip.h).  Everything runs on the GPU
+through libflct_hip.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FlctHipError, Opts, check, f64c, ptr
+
+__all__ = ["Context", "flct", "warp", "shift_frac2d", "parse_skip", "parse_thresh",
+           "FlctHipError"]
+
+
+def parse_thresh(text):
+    """``-t`` value: a trailing 'a' forces absolute units (flct.c:392-404).
+
+    Returns ``(thresh, thresh_abs)``.
+    """
+    s = str(text)
+    absolute = "a" in s
+    if absolute:
+        s = s[: s.index("a")]
+    return _atof(s), absolute
+
+
+def parse_skip(text):
+    """``-s N[pP][qQ][i]`` sub-grammar (flct.c:405-456), parsed right to left.
+
+    Returns ``(skip, poff, qoff, interp)``; raises FlctHipError for the
+    reference's fatal cases (skip <= 0, |p| or |q| >= skip).
+    """
+    s = str(text)
+    interp = False
+    p = q = 0
+    if "i" in s:
+        interp = True
+        s = s[: s.index("i")]
+    if "q" in s:
+        k = s.index("q")
+        q = _atoi(s[k + 1:])
+        s = s[:k]
+    if "p" in s:
+        k = s.index("p")
+        p = _atoi(s[k + 1:])
+        s = s[:k]
+    skip = _atoi(s)
+    if skip <= 0:
+        raise FlctHipError(_lib.FLCT_HIP_E_SKIP_NONPOS, "parse_skip")
+    if abs(p) >= skip or abs(q) >= skip:
+        raise FlctHipError(_lib.FLCT_HIP_E_PQ_RANGE, "parse_skip")
+    return skip, p, q, interp
+
+
+def _atoi(s):
+    """C atoi: optional sign then leading digits, 0 when none."""
+    s = s.strip()
+    k = 0
+    if k < len(s) and s[k] in "+-":
+        k += 1
+    while k < len(s) and s[k].isdigit():
+        k += 1
+    try:
+        return int(s[:k])
+    except ValueError:
+        return 0
+
+
+def _atof(s):
+    """C atof: the longest leading float literal, 0.0 when none."""
+    s = s.strip()
+    for end in range(len(s), 0, -1):
+        try:
+            return float(s[:end])
+        except ValueError:
+            continue
+    return 0.0
+
+
+class Context:
+    """One GPU, one HIP stream, device buffers reused across calls."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        check(_lib.lib().flct_hip_create(ctypes.byref(self._h), int(device)), "flct_hip_create")
+
+    def close(self):
+        if self._h:
+            _lib.lib().flct_hip_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ------------------------------------------------------------ options
+    @staticmethod
+    def make_opts(deltat, deltas, sigma, thresh=0.0, thresh_abs=False, kr=None, skip=0,
+                  poff=0, qoff=0, interp=False, bias=False, precision="fp64", quiet=True):
+        o = Opts()
+        _lib.lib().flct_hip_opts_default(ctypes.byref(o))
+        o.deltat, o.deltas, o.sigma = float(deltat), float(deltas), float(sigma)
+        o.thresh, o.thresh_abs = float(thresh), int(bool(thresh_abs))
+        o.skip, o.poff, o.qoff, o.interp = int(skip or 0), int(poff), int(qoff), int(bool(interp))
+        o.filter = int(kr is not None)
+        o.kr = float(kr or 0.0)
+        o.bias = int(bool(bias))
+        o.precision = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        o.quiet = int(bool(quiet))
+        return o
+
+    # ------------------------------------------------------------ tracking
+    def track(self, f1, f2, deltat, deltas, sigma, **kw):
+        """Track on host arrays; returns (vx, vy, vm) float64 arrays."""
+        f1, f2 = f64c(f1), f64c(f2)
+        if f1.ndim != 2 or f1.shape != f2.shape:
+            raise ValueError("f1 and f2 must be 2-D arrays of the same shape")
+        nx, ny = f1.shape
+        vx, vy, vm = (np.empty((nx, ny)) for _ in range(3))
+        o = self.make_opts(deltat, deltas, sigma, **kw)
+        check(_lib.lib().flct_hip_track(self._h, ptr(f1), ptr(f2), nx, ny, ctypes.byref(o),
+                                        ptr(vx), ptr(vy), ptr(vm)), "flct_hip_track")
+        return vx, vy, vm
+
+    def track_device(self, d_f1, d_f2, nx, ny, d_vx, d_vy, d_vm, opts):
+        check(_lib.lib().flct_hip_track_device(self._h, d_f1, d_f2, nx, ny, ctypes.byref(opts),
+                                               d_vx, d_vy, d_vm), "flct_hip_track_device")
+
+    def last_stats(self):
+        n = ctypes.c_longlong()
+        k = ctypes.c_double()
+        t = ctypes.c_double()
+        check(_lib.lib().flct_hip_last_stats(self._h, ctypes.byref(n), ctypes.byref(k),
+                                             ctypes.byref(t)))
+        return {"tracked": n.value, "kernel_ms": k.value, "total_ms": t.value}
+
+    # ------------------------------------------------------------ warp
+    def warp(self, arr, delx, dely, transp=True):
+        arr, delx, dely = f64c(arr), f64c(delx), f64c(dely)
+        if delx.shape != arr.shape or dely.shape != arr.shape:
+            raise FlctHipError(_lib.FLCT_HIP_E_SHAPE, "warp")
+        nx, ny = arr.shape
+        out = np.empty((nx, ny))
+        check(_lib.lib().flct_hip_warp(self._h, ptr(arr), ptr(delx), ptr(dely), ptr(out), nx,
+                                       ny, int(bool(transp)), 1), "flct_hip_warp")
+        return out
+
+    def shift(self, arr, delx, dely, transp=True):
+        arr = f64c(arr)
+        nx, ny = arr.shape
+        out = np.empty((nx, ny))
+        check(_lib.lib().flct_hip_shift(self._h, ptr(arr), float(delx), float(dely), ptr(out),
+                                        nx, ny, int(bool(transp)), 1), "flct_hip_shift")
+        return out
+
+    # ------------------------------------------------------------ memory
+    def malloc(self, nbytes):
+        p = ctypes.c_void_p()
+        check(_lib.lib().flct_hip_malloc(self._h, ctypes.byref(p), int(nbytes)), "flct_hip_malloc")
+        return p
+
+    def free(self, p):
+        check(_lib.lib().flct_hip_free(self._h, p), "flct_hip_free")
+
+    def upload(self, dptr, a):
+        a = np.ascontiguousarray(a)
+        check(_lib.lib().flct_hip_upload(self._h, dptr, a.ctypes.data_as(ctypes.c_void_p),
+                                         a.nbytes), "flct_hip_upload")
+
+    def download(self, a, dptr):
+        check(_lib.lib().flct_hip_download(self._h, a.ctypes.data_as(ctypes.c_void_p), dptr,
+                                           a.nbytes), "flct_hip_download")
+        return a
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+def flct(f1, f2, deltat, deltas, sigma, thresh=0.0, thresh_abs=False, kr=None, skip=0, poff=0,
+         qoff=0, interp=False, bias=False, precision="fp64", ctx=None):
+    """Local correlation tracking of f2 against f1 (C-space arrays).
+
+    Returns (vx, vy, vm) with vx along the column index j (IDL x) and vy
+    along the row index i (IDL y), in units of deltas/deltat (flct.c:901-905).
+    """
+    ctx = ctx or default_context()
+    return ctx.track(f1, f2, deltat, deltas, sigma, thresh=thresh, thresh_abs=thresh_abs, kr=kr,
+                     skip=skip, poff=poff, qoff=qoff, interp=interp, bias=bias,
+                     precision=precision)
+
+
+def warp(arr, delx, dely, transp=True, ctx=None):
+    """warp_frac2d: evaluate arr's Fourier series at displaced coordinates."""
+    return (ctx or default_context()).warp(arr, delx, dely, transp)
+
+
+def shift_frac2d(arr, delx, dely, transp=True, ctx=None):
+    """Uniform sub-pixel Fourier shift (warp.c:757-881)."""
+    return (ctx or default_context()).shift(arr, delx, dely, transp)

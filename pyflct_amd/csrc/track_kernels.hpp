@@ -1,0 +1,390 @@
+// FLCT per-pixel tracker kernel for gfx950.
+//
+// One "slot" (N/2 quads = 2N lanes) tracks one pixel at a time; a block holds
+// P slots and loops persistently over the tracked-pixel list.  For the box
+// size N = 4*M the three phases per pixel are:
+//
+//  1. column transforms along the box rows ii: each lane loads its box
+//     values, windows them g = G*(f - fbar) (flct.c:856-870, fbar from the
+//     box-mean kernel, flct.c:815-827), both images packed as g1 + i g2; the
+//     transform runs in registers and Z[k][c] goes to LDS;
+//  2. row transforms along jj of the unpacked spectra A = r2c(g1),
+//     B = r2c(g2) for k = 0..N/2 (rows k = 0 and k = N/2 packed into one
+//     transform), the optional k-space filter on both (flct.c:1445-1457),
+//     the cross power conj(A) B normfac (flct.c:1461-1477) and the inverse
+//     transform along jj, written back in place as Q[k][jj];
+//  3. column c2r along k for two columns per transform, |.|, the fftshift
+//     by (N/2, N/2) as index arithmetic, the first-strict-maximum argmax
+//     (flct.c:1512-1534, 2142-2161), the 3x3 Taylor fit (flct.c:1536-1556,
+//     1639-1640) and vx, vy (flct.c:901-905).
+//
+// With bias correction on, the pixel is tracked a second time with the
+// Gaussian windows of image 1 and image 2 centred at -d/2 and +d/2 about the
+// pixel, d being the first estimate (window-offset correction).
+#pragma once
+
+#include "fft_quad.hpp"
+
+namespace flct {
+
+struct TrackArgs {
+  const double* f1;
+  const double* f2;
+  int nx, ny;
+  int nsize;            // box is 2*nsize
+  double sigminv;       // 1/sigma
+  double deltinv;       // 1/deltat
+  double deltas;
+  const double* gw;     // gw[d + N - 1] = exp(-(sigminv d)^2), d = -(N-1)..N-1
+  const double* mean1;  // box means by box origin, (nx-N+1) x (ny-N+1)
+  const double* mean2;
+  const double* filt;   // (N/2+1) x N filter values, or nullptr
+  const int* list;      // tracked pixel list, or nullptr for all pixels in order
+  long long nwork;
+  int bias;
+  double* vx;
+  double* vy;
+  double* vm;
+};
+
+// slots per block for box size 4*M, real type T
+template <int M, class T>
+struct TrackCfg {
+  static constexpr int N = 4 * M;
+  static constexpr int kSlotThreads = 2 * N;
+  static constexpr int kRow = N + 1;  // padded LDS row (complex elements)
+  static constexpr int kZBytes = N * kRow * int(sizeof(cplx<T>));
+  static constexpr int kSlotBytes = ((kZBytes + 4 * N * 8 + 9 * 8 + 32) + 15) / 16 * 16;
+  static constexpr int waste(int p) {
+    int t = p * kSlotThreads;
+    int r = ((t + 63) / 64) * 64;
+    return (r - t) * 1000 / r;
+  }
+  static constexpr int pick() {
+    int best = 1, bw = waste(1);
+    for (int p = 2; p <= 8; ++p) {
+      if (p * kSlotBytes > 80 * 1024 || p * kSlotThreads > 512) break;
+      int w = waste(p);
+      if (w < bw) { best = p; bw = w; }
+    }
+    return best;
+  }
+  static constexpr int P = pick();
+  static constexpr int kThreads = ((P * kSlotThreads + 63) / 64) * 64;
+  static constexpr int kLdsBytes = P * kSlotBytes + (2 * N) * 8;
+};
+
+__device__ __forceinline__ unsigned long long dbits(double v) {
+  return (unsigned long long)__double_as_longlong(v);
+}
+
+// keeps the compiler from moving LDS accesses across this point; LDS
+// requests of one wave are served in issue order, so lanes of a quad see
+// each other's earlier writes
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// an opaque copy of a lane value: address arithmetic derived from it stays
+// inside the phase instead of being hoisted out of the persistent loop,
+// where every distinct address would hold a register for the whole loop
+__device__ __forceinline__ int fresh(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+template <class T>
+__device__ __forceinline__ cplx<T> cross_power(cplx<T> A, cplx<T> B, T norm) {
+  return {(A.re * B.re + A.im * B.im) * norm, (A.re * B.im - A.im * B.re) * norm};
+}
+
+template <int M, class T>
+__global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs a) {
+  using Cfg = TrackCfg<M, T>;
+  constexpr int N = 4 * M;
+  constexpr int R = Cfg::kRow;
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int tid = threadIdx.x;
+  const int slot = tid / Cfg::kSlotThreads;
+  const int stid = tid - slot * Cfg::kSlotThreads;
+  const bool in_slot = slot < Cfg::P;
+  const int lane = stid & 3, qd = stid >> 2;
+  double* gw = reinterpret_cast<double*>(lds);
+  for (int t = tid; t < 2 * N - 1; t += blockDim.x) gw[t] = a.gw[t];
+  unsigned char* base = lds + 2 * N * 8 + (in_slot ? slot : 0) * Cfg::kSlotBytes;
+  cplx<T>* Z = reinterpret_cast<cplx<T>*>(base);
+  double* wx1 = reinterpret_cast<double*>(base + Cfg::kZBytes);
+  double* wy1 = wx1 + N;
+  double* wx2 = wy1 + N;
+  double* wy2 = wx2 + N;
+  double* nb = wy2 + N;
+  double* res = nb + 9;
+  unsigned long long* amax = reinterpret_cast<unsigned long long*>(res + 2);
+  unsigned int* aidx = reinterpret_cast<unsigned int*>(amax + 1);
+  __syncthreads();
+
+  const int nx = a.nx, ny = a.ny, nsize = a.nsize;
+  const int ns = ny - N + 1;
+  const T norm = T((1. / ((double)N * N)) * (1. / ((double)N * N)));
+  const long long nslots = (long long)gridDim.x * Cfg::P;
+  const long long first = (long long)blockIdx.x * Cfg::P + slot;
+  const long long iters = (a.nwork + nslots - 1) / nslots;
+  const int npass = a.bias ? 2 : 1;
+
+  for (long long it = 0; it < iters; ++it) {
+    const long long w = first + it * nslots;
+    const bool active = in_slot && w < a.nwork;
+    int i = 0, j = 0, imin = 0, jmin = 0;
+    double f1bar = 0., f2bar = 0.;
+    if (active) {
+      const long long p = a.list ? (long long)a.list[w] : w;
+      i = (int)(p / ny);
+      j = (int)(p - (long long)i * ny);
+      // box clamp, flct.c:773-786
+      const int imin0 = (0 > (i - (nsize - 1))) ? 0 : i - (nsize - 1);
+      const int imax0 = ((nx - 1) < (i + nsize)) ? nx - 1 : i + nsize;
+      imin = (imax0 == nx - 1) ? nx - 1 - (2 * nsize - 1) : imin0;
+      const int jmin0 = (0 > (j - (nsize - 1))) ? 0 : j - (nsize - 1);
+      const int jmax0 = ((ny - 1) < (j + nsize)) ? ny - 1 : j + nsize;
+      jmin = (jmax0 == ny - 1) ? ny - 1 - (2 * nsize - 1) : jmin0;
+      f1bar = a.mean1[(long long)imin * ns + jmin];
+      f2bar = a.mean2[(long long)imin * ns + jmin];
+    }
+    double shiftx = 0., shifty = 0.;
+    for (int pass = 0; pass < npass; ++pass) {
+      // ------------- windows: separable Gaussian centred on the pixel, or
+      // moved by -d/2 (image 1) and +d/2 (image 2) in the bias pass
+      if (active) {
+        const double cx = 0.5 * shiftx, cy = 0.5 * shifty;
+        if (stid < N) {
+          const int d = stid + imin - i;
+          if (pass == 0) {
+            const double g = gw[d + N - 1];
+            wx1[stid] = g;
+            wx2[stid] = g;
+          } else {
+            const double a1 = a.sigminv * ((double)d + cx), a2 = a.sigminv * ((double)d - cx);
+            wx1[stid] = exp(-a1 * a1);
+            wx2[stid] = exp(-a2 * a2);
+          }
+        } else {
+          const int c = stid - N;
+          const int d = c + jmin - j;
+          if (pass == 0) {
+            const double g = gw[d + N - 1];
+            wy1[c] = g;
+            wy2[c] = g;
+          } else {
+            const double b1 = a.sigminv * ((double)d + cy), b2 = a.sigminv * ((double)d - cy);
+            wy1[c] = exp(-b1 * b1);
+            wy2[c] = exp(-b2 * b2);
+          }
+        }
+        if (stid == 0) {
+          *amax = 0ull;
+          *aidx = 0xffffffffu;
+        }
+      }
+      __syncthreads();
+      // ------------- phase 1: column transforms, one column at a time
+      if (active) {
+        const int l = fresh(lane), quad = fresh(qd);
+        const double* p1 = a.f1 + (long long)imin * ny + jmin;
+        const double* p2 = a.f2 + (long long)imin * ny + jmin;
+        const int kb = M * brev2(l);
+#pragma unroll 1
+        for (int t = 0; t < 2; ++t) {
+          const int c = quad + t * (N / 2);
+          const double w1c = wy1[c], w2c = wy2[c];
+          cplx<T> z[M];
+          static_for<M>([&](auto Rr) {
+            const int ii = 4 * Rr + l;
+            const long long off = (long long)ii * ny + c;
+            const double x1 = p1[off], x2 = p2[off];
+            z[Rr] = {T((wx1[ii] * w1c) * (x1 - f1bar)), T((wx2[ii] * w2c) * (x2 - f2bar))};
+          });
+          quad_dit<M, -1>(z, l);
+          static_for<M>([&](auto S) { Z[(kb + S) * R + c] = z[S]; });
+        }
+      }
+      __syncthreads();
+      // ------------- phase 2: row transforms, cross power, inverse along jj
+      if (active) {
+        const int l = fresh(lane), quad = fresh(qd);
+        const int t = quad;
+        const int lb = M * brev2(l);
+        if (t >= 1) {
+          const int k = t, km = N - t;
+          cplx<T> u1[M], u2[M];
+          static_for<M>([&](auto Rr) {
+            const int jj = 4 * Rr + l;
+            const cplx<T> zk = Z[k * R + jj], zm = Z[km * R + jj];
+            // A = (Zk + conj(Zm))/2, B = (Zk - conj(Zm))/(2i)
+            u1[Rr] = {T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
+            u2[Rr] = {T(0.5) * (zk.im + zm.im), T(-0.5) * (zk.re - zm.re)};
+          });
+          quad_dit<M, -1>(u1, l);
+          // park A's spectrum in the mirror row, which only this quad reads
+          wave_lds_order();
+          static_for<M>([&](auto S) { Z[km * R + lb + S] = u1[S]; });
+          wave_lds_order();
+          quad_dit<M, -1>(u2, l);
+          wave_lds_order();
+          static_for<M>([&](auto S) {
+            cplx<T> A = Z[km * R + lb + S], B = u2[S];
+            if (a.filt) {
+              const T f = T(a.filt[k * N + lb + S]);
+              A = {A.re * f, A.im * f};
+              B = {B.re * f, B.im * f};
+            }
+            u2[S] = cross_power(A, B, norm);
+          });
+          quad_dif<M, +1>(u2, l);
+          static_for<M>([&](auto Rr) { Z[k * R + 4 * Rr + l] = u2[Rr]; });
+        } else {
+          // rows k = 0 and k = N/2 of A and B are real along jj: one packed
+          // transform per image, parked in rows N/2 (A) and 0 (B)
+          cplx<T> u[M], v[M];
+          static_for<M>([&](auto Rr) {
+            const int jj = 4 * Rr + l;
+            const cplx<T> z0 = Z[0 * R + jj], zh = Z[(N / 2) * R + jj];
+            u[Rr] = {z0.re, zh.re};
+            v[Rr] = {z0.im, zh.im};
+          });
+          quad_dit<M, -1>(u, l);
+          wave_lds_order();
+          static_for<M>([&](auto S) { Z[(N / 2) * R + lb + S] = u[S]; });
+          wave_lds_order();
+          quad_dit<M, -1>(v, l);
+          wave_lds_order();
+          static_for<M>([&](auto S) { Z[0 * R + lb + S] = v[S]; });
+          wave_lds_order();
+          static_for<M>([&](auto S) {
+            const int lx = lb + S;
+            const int lm = (N - lx) % N;
+            const cplx<T> U = Z[(N / 2) * R + lx], Um = Z[(N / 2) * R + lm];
+            const cplx<T> V = Z[0 * R + lx], Vm = Z[0 * R + lm];
+            // row 0 = (U + conj(Um))/2, row N/2 = (U - conj(Um))/(2i)
+            cplx<T> A0 = {T(0.5) * (U.re + Um.re), T(0.5) * (U.im - Um.im)};
+            cplx<T> Ah = {T(0.5) * (U.im + Um.im), T(-0.5) * (U.re - Um.re)};
+            cplx<T> B0 = {T(0.5) * (V.re + Vm.re), T(0.5) * (V.im - Vm.im)};
+            cplx<T> Bh = {T(0.5) * (V.im + Vm.im), T(-0.5) * (V.re - Vm.re)};
+            if (a.filt) {
+              const T f0 = T(a.filt[0 * N + lx]), fh = T(a.filt[(N / 2) * N + lx]);
+              A0 = {A0.re * f0, A0.im * f0};
+              B0 = {B0.re * f0, B0.im * f0};
+              Ah = {Ah.re * fh, Ah.im * fh};
+              Bh = {Bh.re * fh, Bh.im * fh};
+            }
+            const cplx<T> P0 = cross_power(A0, B0, norm), Ph = cross_power(Ah, Bh, norm);
+            u[S] = {P0.re - Ph.im, P0.im + Ph.re};
+          });
+          quad_dif<M, +1>(u, l);
+          wave_lds_order();
+          static_for<M>([&](auto Rr) {
+            const int jj = 4 * Rr + l;
+            Z[0 * R + jj] = {u[Rr].re, T(0)};
+            Z[(N / 2) * R + jj] = {u[Rr].im, T(0)};
+          });
+        }
+      }
+      __syncthreads();
+      // ------------- phase 3: c2r along k, two columns per transform
+      T va[M], vb[M];
+      const int l = fresh(lane), quad = fresh(qd);
+      const int ca = 2 * quad, cb = 2 * quad + 1;
+      const int ib = M * brev2(l);
+      const int sa = (ca + N / 2) % N, sb = (cb + N / 2) % N;
+      if (active) {
+        cplx<T> z[M];
+        static_for<M>([&](auto Rr) {
+          const int k = 4 * Rr + l;
+          cplx<T> qa, qb;
+          if (k <= N / 2) {
+            qa = Z[k * R + ca];
+            qb = Z[k * R + cb];
+          } else {
+            qa = cconj(Z[(N - k) * R + ca]);
+            qb = cconj(Z[(N - k) * R + cb]);
+          }
+          z[Rr] = {qa.re - qb.im, qa.im + qb.re};
+        });
+        quad_dit<M, +1>(z, l);
+        double lmax = 0.;
+        static_for<M>([&](auto S) {
+          va[S] = fabs(z[S].re);
+          vb[S] = fabs(z[S].im);
+          lmax = fmax(lmax, (double)va[S]);
+          lmax = fmax(lmax, (double)vb[S]);
+        });
+        lmax = fmax(lmax, dpp<kQuadXor1>(lmax));
+        lmax = fmax(lmax, dpp<kQuadXor2>(lmax));
+        if (l == 0) atomicMax(amax, dbits(lmax));
+      }
+      __syncthreads();
+      const double gmax = __longlong_as_double((long long)*amax);
+      if (active) {
+        unsigned int best = 0xffffffffu;
+        static_for<M>([&](auto S) {
+          const int si = ((ib + S + N / 2) % N) * N;
+          if ((double)va[S] == gmax) best = min(best, (unsigned)(si + sa));
+          if ((double)vb[S] == gmax) best = min(best, (unsigned)(si + sb));
+        });
+        best = min(best, (unsigned)__builtin_amdgcn_mov_dpp((int)best, kQuadXor1, 0xF, 0xF, true));
+        best = min(best, (unsigned)__builtin_amdgcn_mov_dpp((int)best, kQuadXor2, 0xF, 0xF, true));
+        if (l == 0 && best != 0xffffffffu) atomicMin(aidx, best);
+      }
+      __syncthreads();
+      const unsigned loc = *aidx;
+      const bool nonzero = (gmax != 0.);
+      int ixmax = N / 2, iymax = N / 2;
+      if (nonzero) {
+        ixmax = (int)(loc / N);
+        iymax = (int)(loc % N);
+      }
+      const bool fit = nonzero && ixmax > 0 && ixmax < N - 1 && iymax > 0 && iymax < N - 1;
+      if (active && fit) {
+        static_for<M>([&](auto S) {
+          const int si = (ib + S + N / 2) % N;
+          const int di = si - ixmax;
+          if (di >= -1 && di <= 1) {
+            const int da = sa - iymax, db = sb - iymax;
+            if (da >= -1 && da <= 1) nb[(di + 1) * 3 + (da + 1)] = (double)va[S];
+            if (db >= -1 && db <= 1) nb[(di + 1) * 3 + (db + 1)] = (double)vb[S];
+          }
+        });
+      }
+      __syncthreads();
+      if (active && stid == 0) {
+#pragma clang fp contract(off)
+        double shiftsubx = 0., shiftsuby = 0.;
+        if (fit) {
+          // nb[(dx+1)*3 + (dy+1)] = surface at (ixmax+dx, iymax+dy)
+          const double c00 = nb[4];
+          const double fx = 0.5 * (nb[7] - nb[1]);
+          const double fy = 0.5 * (nb[5] - nb[3]);
+          const double fxx = (nb[7] + nb[1] - 2. * (c00));
+          const double fyy = (nb[5] + nb[3] - 2. * (c00));
+          const double fxy = 0.25 * (nb[8] + nb[0] - nb[6] - nb[2]);
+          shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
+          shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
+        }
+        res[0] = ((double)ixmax + shiftsubx) - (double)(N / 2);
+        res[1] = ((double)iymax + shiftsuby) - (double)(N / 2);
+      }
+      __syncthreads();
+      shiftx = res[0];
+      shifty = res[1];
+    }
+    if (active && stid == 0) {
+      const long long p = (long long)i * ny + j;
+      a.vx[p] = shifty * a.deltinv * a.deltas;
+      a.vy[p] = shiftx * a.deltinv * a.deltas;
+      a.vm[p] = 1.;
+    }
+  }
+}
+
+}  // namespace flct
