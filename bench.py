@@ -1,0 +1,198 @@
+"""Benchmark: tracked pixels per second on the 4096^2, sigma = 15 full-disk
+pair (64 x 64 boxes, every pixel tracked), one MI355X.
+
+One step = one flct_hip_track_device call over the resident image pair:
+box means, the tracker kernel over all 16,777,216 pixels, and the final
+reset.  Inputs are in HBM before the timed region.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "tracked px/s at 4096^2 sigma=15: split-BF16 MFMA vs FP64 plain; RMS vel err"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
+PMC_FILE = os.path.join(ROOT, "profiles", "track_f64_4096_sigma15_traffic.json")
+
+
+def w_fft(n):
+    """Algorithmic flops per tracked pixel of the FFT form: three real 2-D
+    FFTs (2.5 N^2 log2 N^2 each) plus window, mean, filter, product, |.|,
+    argmax: 15 N^2 log2 N + 14 N^2."""
+    return 15.0 * n * n * math.log2(n) + 14.0 * n * n
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(f1, f2, sigma, budget_s=20.0):
+    """The CPU oracle (single-thread C restatement of the reference loop) on a
+    band of rows through the disk centre; returns (px/s, sample text)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    n = f1.shape[1]
+    c = f1.shape[0] // 2
+    probe = np.arange(c * n + n // 2 - 256, c * n + n // 2 + 256)
+    t = time.perf_counter()
+    oracle_lib.flct_pixels(f1, f2, 1.0, 1.0, sigma, probe)
+    per_px = (time.perf_counter() - t) / probe.size
+    rows = int(max(1, min(32, round(budget_s / per_px / n))))
+    r0 = c - rows // 2
+    pix = np.arange(r0 * n, (r0 + rows) * n)
+    t = time.perf_counter()
+    oracle_lib.flct_pixels(f1, f2, 1.0, 1.0, sigma, pix)
+    dt = time.perf_counter() - t
+    return pix.size / dt, f"rows {r0}-{r0 + rows - 1} of the 4096^2 sigma=15 pair ({pix.size} px), 1 thread"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--sigma", type=float, default=15.0)
+    ap.add_argument("--forms", default="fp64,fp32,split_bf16")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not supported "
+              "(the tracker shards by row strips, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from pyflct_amd import api, synth
+    from pyflct_amd._lib import FlctHipError
+
+    log(f"bench: generating the {args.size}^2 pair")
+    f1, f2, vtrue = synth.cfg4(n=args.size)
+    nx, ny = f1.shape
+    ctx = api.Context(0)
+    nbytes = f1.nbytes
+    d = {k: ctx.malloc(nbytes) for k in ("f1", "f2", "vx", "vy", "vm")}
+    ctx.upload(d["f1"], f1)
+    ctx.upload(d["f2"], f2)
+
+    forms = [f for f in args.forms.split(",") if f]
+    results = {}
+    ref = None
+    for form in forms:
+        opts = ctx.make_opts(1.0, 1.0, args.sigma, precision=form)
+        try:
+            for _ in range(args.warmup):
+                ctx.track_device(d["f1"], d["f2"], nx, ny, d["vx"], d["vy"], d["vm"], opts)
+        except FlctHipError as e:
+            log(f"bench: form {form} not available: {e}")
+            continue
+        log(f"bench: timing {form}, {args.steps} steps")
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.track_device(d["f1"], d["f2"], nx, ny, d["vx"], d["vy"], d["vm"], opts)
+            kms.append(ctx.last_stats()["kernel_ms"])
+        t1 = time.perf_counter()
+        st = ctx.last_stats()
+        ms = (t1 - t0) * 1e3 / args.steps
+        vx = ctx.download(np.empty_like(f1), d["vx"])
+        vy = ctx.download(np.empty_like(f1), d["vy"])
+        results[form] = {"ms_per_step": ms, "tracked": st["tracked"],
+                         "px_per_s": st["tracked"] / (ms * 1e-3),
+                         "kernel_ms": float(np.mean(kms)), "vx": vx, "vy": vy}
+        if form == "fp64":
+            ref = (vx, vy)
+        log(f"bench: {form}: {ms:.1f} ms/step, kernel {np.mean(kms):.1f} ms, "
+            f"{results[form]['px_per_s'] / 1e6:.2f} Mpx/s")
+
+    if "fp64" not in results:
+        raise SystemExit("bench: the FP64 plain form failed")
+    nbox = ctx.box_size(nx, ny, args.sigma)
+    for form, r in results.items():
+        r["rms_vel_err_vs_fp64"] = float(np.sqrt(np.mean((r["vx"] - ref[0]) ** 2 +
+                                                         (r["vy"] - ref[1]) ** 2)))
+        r["max_vel_err_vs_fp64"] = float(max(np.max(np.abs(r["vx"] - ref[0])),
+                                             np.max(np.abs(r["vy"] - ref[1]))))
+
+    # sampled parity of the FP64 form against the CPU oracle
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    rng = np.random.default_rng(1)
+    pix = rng.choice(nx * ny, 64, replace=False)
+    ox, oy, _ = oracle_lib.flct_pixels(f1, f2, 1.0, 1.0, args.sigma, pix)
+    parity = float(max(np.max(np.abs(ref[0].ravel()[pix] - ox)),
+                       np.max(np.abs(ref[1].ravel()[pix] - oy))))
+
+    # headline: the split-BF16 form when it meets the 1e-4 px RMS bound
+    head = "fp64"
+    if "split_bf16" in results and results["split_bf16"]["rms_vel_err_vs_fp64"] <= 1e-4:
+        head = "split_bf16"
+    hr = results[head]
+    dtype = {"fp64": "f64", "fp32": "f32", "split_bf16": "bf16x3+f32acc"}[head]
+
+    flops = w_fft(nbox) * hr["tracked"]
+    if head == "fp64":
+        peak, bound = PEAK_FP64_TFLOPS, "mfma"
+    elif head == "fp32":
+        peak, bound = PEAK_FP32_TFLOPS, "mfma"
+    else:
+        peak, bound = PEAK_BF16_TFLOPS, "mfma"
+    achieved = flops / (hr["kernel_ms"] * 1e-3) / 1e12
+    traffic = None
+    if os.path.exists(PMC_FILE):
+        try:
+            traffic = json.load(open(PMC_FILE)).get(head, {}).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    roof = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": traffic,
+            "kernel": f"k_track<{nbox // 4},{'double' if head == 'fp64' else 'float'}>",
+            "flops_per_px": w_fft(nbox), "kernel_ms": round(hr["kernel_ms"], 3)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: timing the CPU oracle baseline")
+        v, sample = cpu_baseline(f1, f2, args.sigma, args.cpu_budget)
+        cpu = {"value": round(v, 1), "unit": "tracked px/s", "cores": 1, "kind": "port",
+               "sample": sample}
+
+    out = {
+        "metric": METRIC,
+        "value": round(hr["px_per_s"], 1),
+        "unit": "tracked px/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(hr["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (numpy PCG64 seed 20261206 full-disk magnetogram pair)",
+        "config": {"workload": f"full-disk pair {nx}x{ny}, sigma={args.sigma:g} "
+                               f"({nbox}x{nbox} boxes), all pixels tracked",
+                   "form": head, "box": nbox, "tracked_px": hr["tracked"]},
+        "forms": {f: {"px_per_s": round(r["px_per_s"], 1), "ms_per_step": round(r["ms_per_step"], 3),
+                      "kernel_ms": round(r["kernel_ms"], 3),
+                      "rms_vel_err_vs_fp64": r["rms_vel_err_vs_fp64"],
+                      "max_vel_err_vs_fp64": r["max_vel_err_vs_fp64"]}
+                  for f, r in results.items()},
+        "fp64_vs_oracle_max_err_px": parity,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    for k in d.values():
+        ctx.free(k)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,11 @@ void flct_hip_opts_default(flct_hip_opts *opts);
 /* Human-readable text for a status code. */
 const char *flct_hip_strerror(int status);
 
+/* Validate opts for an nx x ny image on the host (no device needed): the
+ * reference's option and box-size checks (flct.c:459-472, 559-563, 575-579).
+ * On success *nbox (if non-null) receives the sliding-box size 2*nsize. */
+int flct_hip_check_opts(int nx, int ny, const flct_hip_opts *opts, int *nbox);
+
 /* Create a context on HIP device `device`.  Allocates nothing large. */
 int flct_hip_create(flct_hip_ctx **ctx, int device);
 int flct_hip_destroy(flct_hip_ctx *ctx);

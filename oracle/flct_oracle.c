@@ -25,6 +25,10 @@
  *     parity unpinned (only invariants check it).
  *
  * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).
+ *
+ * PLANT_* macros (never set in the normal build) compile deliberately wrong
+ * variants; the test suite builds them to show that the pins above fail
+ * on a plausible mistake.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -188,6 +192,16 @@ int oracle_make_freq(double *k, int ndim) {
   int n21 = (ndim / 2) - 1, i, inext;
   for (i = 0; i < n21 + 1; i++) k[i] = (double)i;
   inext = n21 + 1;
+#ifdef PLANT_NYQUIST_NEG
+  if (1) {
+    k[inext] = -(double)(ndim / 2);
+    inext++;
+    if ((ndim / 2) * 2 != ndim) {
+      k[inext] = -(double)(ndim / 2);
+      inext++;
+    }
+  } else
+#endif
   if ((ndim / 2) * 2 != ndim) {
     k[inext] = (double)(ndim / 2);
     inext++;
@@ -321,7 +335,9 @@ static void xcor_surface(xcor_ws_t *w, const double *arr, const double *barr) {
         /* flct.c:1445-1457: both spectra multiplied by filter[i*ny+j] */
         double f = w->filter[i * ny + j];
         ar = ar * f; ai = ai * f;
+#ifndef PLANT_FILTER_ONCE
         br = br * f; bi = bi * f;
+#endif
       }
       /* flct.c:1461-1477 */
       cr = (ar * br + ai * bi) * normfac;
@@ -357,7 +373,16 @@ static void xcor_peak(const double *a, int nx, int ny, double *shiftx,
                       double *shifty) {
   int maxind, ixmax, iymax, absccmax = 1;
   double shiftx0, shifty0, shiftsubx = 0., shiftsuby = 0.;
+#ifdef PLANT_TIE_LAST
+  {
+    int q;
+    maxind = 0;
+    for (q = 1; q < nx * ny; q++)
+      if (a[q] >= a[maxind]) maxind = q;
+  }
+#else
   maxind = maxloc(a, nx * ny);
+#endif
   if (a[maxind] == (double)0.) absccmax = 0;
   if (absccmax == 1) {
     ixmax = maxind / ny;
@@ -379,6 +404,9 @@ static void xcor_peak(const double *a, int nx, int ny, double *shiftx,
            2. * (a[ixmax * ny + iymax]));
     fxy = 0.25 * (a[(ixmax + 1) * ny + iymax + 1] + a[(ixmax - 1) * ny + iymax - 1] -
                   a[(ixmax + 1) * ny + iymax - 1] - a[(ixmax - 1) * ny + iymax + 1]);
+#ifdef PLANT_FIT_SIGN
+    fxy = -fxy;
+#endif
     shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
     shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
   }
@@ -549,6 +577,9 @@ static void track_pixel(track_ctx_t *t, const double *f1, const double *f2, int 
     }
   f1bar = f1bar / ((double)isize * jsize);
   f2bar = f2bar / ((double)isize * jsize);
+#ifdef PLANT_NO_MEAN
+  f1bar = f2bar = 0.;
+#endif
   for (ii = 0; ii < isize; ii++)
     for (jj = 0; jj < jsize; jj++) {
       double g = gauss_off(t, ii + imin - i, jj + jmin - j);
@@ -574,8 +605,13 @@ static void track_pixel(track_ctx_t *t, const double *f1, const double *f2, int 
     xcor_peak(t->ws.shifted, isize, jsize, &shiftx, &shifty);
   }
   /* transp = 1 (flct.c:901-905) */
+#ifdef PLANT_NO_SWAP
+  *vxx = shiftx * deltinv * o->deltas;
+  *vyy = shifty * deltinv * o->deltas;
+#else
   *vxx = shifty * deltinv * o->deltas;
   *vyy = shiftx * deltinv * o->deltas;
+#endif
 }
 
 static int track_ctx_init(track_ctx_t *t, int nx, int ny, const oracle_opts_t *o) {

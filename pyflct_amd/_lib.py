@@ -34,6 +34,7 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "split_bf16": SPLIT_BF16, "bf16x3": SP
 # entry points declared in include/flct_hip.h
 EXPORTS = (
     "flct_hip_opts_default",
+    "flct_hip_check_opts",
     "flct_hip_strerror",
     "flct_hip_create",
     "flct_hip_destroy",
@@ -92,6 +93,7 @@ def lib():
         L.flct_hip_opts_default.restype = None
         L.flct_hip_strerror.argtypes = [i]
         L.flct_hip_strerror.restype = ctypes.c_char_p
+        L.flct_hip_check_opts.argtypes = [i, i, ctypes.POINTER(Opts), ctypes.POINTER(i)]
         L.flct_hip_create.argtypes = [ctypes.POINTER(vp), i]
         L.flct_hip_destroy.argtypes = [vp]
         L.flct_hip_track.argtypes = [vp, _dp, _dp, i, i, ctypes.POINTER(Opts), _dp, _dp, _dp]

@@ -128,6 +128,15 @@ class Context:
         o.quiet = int(bool(quiet))
         return o
 
+    @staticmethod
+    def box_size(nx, ny, sigma):
+        """Sliding-box size 2*nsize for an nx x ny image (flct.c:567-580)."""
+        o = Context.make_opts(1.0, 1.0, sigma)
+        nbox = ctypes.c_int(0)
+        check(_lib.lib().flct_hip_check_opts(int(nx), int(ny), ctypes.byref(o), ctypes.byref(nbox)),
+              "flct_hip_check_opts")
+        return nbox.value
+
     # ------------------------------------------------------------ tracking
     def track(self, f1, f2, deltat, deltas, sigma, **kw):
         """Track on host arrays; returns (vx, vy, vm) float64 arrays."""

@@ -11,12 +11,12 @@
 #include "aux_kernels.hpp"
 #include "track_kernels.hpp"
 #include "warp_kernels.hpp"
+#include "track_launch.hpp"
 
 using namespace flct;
 
 namespace {
 
-constexpr int kMaxM = 24;  // box sizes N = 4..96
 
 struct DevBuf {
   void* p = nullptr;
@@ -164,36 +164,19 @@ int resolve(const flct_hip_opts* o, int nx, int ny, Resolved* r) {
   return FLCT_HIP_OK;
 }
 
-template <int M, class T>
-int launch_track(flct_hip_ctx* c, const TrackArgs& a) {
-  using Cfg = TrackCfg<M, T>;
-  const size_t lds = Cfg::kLdsBytes;
-  if (lds > c->lds_per_block) return FLCT_HIP_E_UNSUPPORTED;
-  if (lds > 65536)
-    (void)hipFuncSetAttribute((const void*)k_track<M, T>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  long long per_cu = (long long)(160 * 1024) / (long long)lds;
-  if (per_cu < 1) per_cu = 1;
-  if (per_cu > 8) per_cu = 8;
-  long long grid = (long long)c->num_cu * per_cu;
-  const long long need = (a.nwork + Cfg::P - 1) / Cfg::P;
-  if (grid > need) grid = need;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((k_track<M, T>), dim3((unsigned)grid), dim3(Cfg::kThreads), lds, c->stream, a);
-  return hipGetLastError() == hipSuccess ? FLCT_HIP_OK : FLCT_HIP_E_HIP;
-}
-
-template <class T>
-int dispatch_track(flct_hip_ctx* c, int M, const TrackArgs& a) {
-  int rc = FLCT_HIP_E_UNSUPPORTED;
-  static_for<kMaxM>([&](auto I) {
-    constexpr int m = I + 1;
-    if (m == M) rc = launch_track<m, T>(c, a);
-  });
-  return rc;
-}
-
 }  // namespace
+
+extern "C" int flct_hip_check_opts(int nx, int ny, const flct_hip_opts* o, int* nbox) {
+  if (!o || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
+  Resolved r;
+  const int rc = resolve(o, nx, ny, &r);
+  if (rc) return rc;
+  if (nbox) *nbox = r.N;
+  if (r.N % 4 != 0 || r.N / 4 > kTrackMaxM) return FLCT_HIP_E_UNSUPPORTED;
+  if (o->precision < FLCT_HIP_FP64 || o->precision > FLCT_HIP_SPLIT_BF16) return FLCT_HIP_E_ARG;
+  if (o->deltat == 0.) return FLCT_HIP_E_ARG;
+  return FLCT_HIP_OK;
+}
 
 static double ev_ms(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
@@ -211,7 +194,7 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
   if (rc) return rc;
   if (o->precision != FLCT_HIP_FP64 && o->precision != FLCT_HIP_FP32) return FLCT_HIP_E_UNSUPPORTED;
   const int N = r.N, M = N / 4;
-  if (N % 4 != 0 || M < 1 || M > kMaxM) return FLCT_HIP_E_UNSUPPORTED;
+  if (N % 4 != 0 || M < 1 || M > kTrackMaxM) return FLCT_HIP_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->device));
   const long long npix = (long long)nx * ny;
   if (npix > 0x7fffffffLL) return FLCT_HIP_E_UNSUPPORTED;
@@ -344,8 +327,11 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
   ta.vm = vm;
   HIPCHK(hipEventRecord(c->ev[1], s));
   if (nwork > 0) {
-    rc = (o->precision == FLCT_HIP_FP32) ? dispatch_track<float>(c, M, ta)
-                                         : dispatch_track<double>(c, M, ta);
+    LaunchCtx lc{c->stream, c->num_cu, c->lds_per_block};
+    if (o->precision == FLCT_HIP_FP32)
+      rc = (M <= 12) ? dispatch_track_f32_lo(M, lc, ta) : dispatch_track_f32_hi(M, lc, ta);
+    else
+      rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, ta) : dispatch_track_f64_hi(M, lc, ta);
     if (rc) return rc;
   }
   HIPCHK(hipEventRecord(c->ev[2], s));

@@ -33,24 +33,47 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        i, d = ctypes.c_int, ctypes.c_double
-        L.oracle_flct.argtypes = [_dp, _dp, i, i, d, d, d, d, i, i, i, i, i, i, d, i,
-                                  _dp, _dp, _dp]
-        L.oracle_flct_pixels.argtypes = [_dp, _dp, i, i, d, d, d, d, i, i, i, i, i,
-                                         d, i, _lp, ctypes.c_longlong, _dp, _dp, _dp]
-        L.oracle_cross_cor.argtypes = [_dp, _dp, i, i, i, d, _dp, _dp, _dp]
-        L.oracle_peak_fit.argtypes = [_dp, i, i, _dp, _dp]
-        L.oracle_interpcc2d.argtypes = [_dp, d, i, i, _dp, i, _dp, i, _dp]
-        L.oracle_fft2.argtypes = [_dp, i, i, i]
-        L.oracle_make_freq.argtypes = [_dp, i]
-        L.oracle_box_nsize.argtypes = [d, i, i]
-        L.oracle_warp_frac2d.argtypes = [_dp, _dp, _dp, _dp, i, i, i]
-        L.oracle_warp_frac2d_rows.argtypes = [_dp, _dp, _dp, _dp, i, i, i, i, i]
-        L.oracle_shift_frac2d.argtypes = [_dp, d, d, _dp, i, i, i]
-        L.oracle_resolve_thresh.argtypes = [_dp, _dp, i, d, i, _dp]
-        _lib = L
+        _lib = _bind(LIB_PATH)
     return _lib
+
+
+class using:
+    """Temporarily route every oracle call to another build of the oracle
+    (used by the planted-mistake tests)."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self.saved = _lib
+        _lib = _bind(self.path)
+        return self
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
+
+
+def _bind(path):
+    L = ctypes.CDLL(path)
+    i, d = ctypes.c_int, ctypes.c_double
+    L.oracle_flct.argtypes = [_dp, _dp, i, i, d, d, d, d, i, i, i, i, i, i, d, i,
+                              _dp, _dp, _dp]
+    L.oracle_flct_pixels.argtypes = [_dp, _dp, i, i, d, d, d, d, i, i, i, i, i,
+                                     d, i, _lp, ctypes.c_longlong, _dp, _dp, _dp]
+    L.oracle_cross_cor.argtypes = [_dp, _dp, i, i, i, d, _dp, _dp, _dp]
+    L.oracle_peak_fit.argtypes = [_dp, i, i, _dp, _dp]
+    L.oracle_interpcc2d.argtypes = [_dp, d, i, i, _dp, i, _dp, i, _dp]
+    L.oracle_fft2.argtypes = [_dp, i, i, i]
+    L.oracle_make_freq.argtypes = [_dp, i]
+    L.oracle_box_nsize.argtypes = [d, i, i]
+    L.oracle_warp_frac2d.argtypes = [_dp, _dp, _dp, _dp, i, i, i]
+    L.oracle_warp_frac2d_rows.argtypes = [_dp, _dp, _dp, _dp, i, i, i, i, i]
+    L.oracle_shift_frac2d.argtypes = [_dp, d, d, _dp, i, i, i]
+    L.oracle_resolve_thresh.argtypes = [_dp, _dp, i, d, i, _dp]
+    return L
 
 
 def _f64(a):

@@ -1,0 +1,64 @@
+"""GPU parity of flct_hip_warp (warp_frac2d) and flct_hip_shift
+(shift_frac2d) against the oracle and the reference fixture."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_warp_fixture(ctx, golden):
+    w = ctx.warp(golden["f1"], golden["dx"], golden["dy"])
+    wo = O.warp_frac2d(golden["f1"], golden["dx"], golden["dy"])
+    assert np.max(np.abs(w - wo)) < 1e-12 * np.max(np.abs(wo)) * 100
+    assert np.max(np.abs(w - golden["f2"])) < 5e-7
+
+
+@pytest.mark.parametrize("shape", [(37, 50), (64, 64), (33, 17), (48, 31), (96, 80)])
+def test_warp_random_fields(ctx, shape):
+    rng = np.random.default_rng(sum(shape))
+    a = rng.standard_normal(shape) * 10.0
+    dx = rng.standard_normal(shape) * 1.5
+    dy = rng.standard_normal(shape) * 1.5
+    for transp in (1, 0):
+        w = ctx.warp(a, dx, dy, transp=bool(transp))
+        wo = O.warp_frac2d(a, dx, dy, transp=transp)
+        assert np.max(np.abs(w - wo)) < 1e-10 * np.max(np.abs(a))
+
+
+def test_warp_integer_shift_is_roll(ctx):
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((60, 72))
+    dx = np.full(a.shape, 3.0)   # IDL x = columns j
+    dy = np.full(a.shape, -2.0)  # IDL y = rows i
+    w = ctx.warp(a, dx, dy)
+    assert np.max(np.abs(w - np.roll(np.roll(a, -2, axis=0), 3, axis=1))) < 1e-11
+
+
+def test_uniform_warp_equals_shift(ctx):
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((45, 64))
+    w = ctx.warp(a, np.full(a.shape, 0.37), np.full(a.shape, -1.21))
+    s = ctx.shift(a, 0.37, -1.21)
+    assert np.max(np.abs(w - s)) < 1e-11
+
+
+@pytest.mark.parametrize("shape", [(64, 48), (101, 201), (31, 31)])
+def test_shift_matches_oracle(ctx, shape):
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(shape)
+    s = ctx.shift(a, 0.3, -0.2)
+    so = O.shift_frac2d(a, 0.3, -0.2)
+    assert np.max(np.abs(s - so)) < 1e-12
+
+
+def test_warp_large_properties(ctx):
+    """512^2: the zero field is the identity, an integer field is a roll."""
+    rng = np.random.default_rng(4)
+    a = rng.laplace(0, 50, (512, 512))
+    z = np.zeros_like(a)
+    w = ctx.warp(a, z, z)
+    assert np.max(np.abs(w - a)) < 1e-9 * np.max(np.abs(a))
+    w = ctx.warp(a, z + 5.0, z + 1.0)
+    assert np.max(np.abs(w - np.roll(np.roll(a, 1, axis=0), 5, axis=1))) < 1e-9 * np.max(np.abs(a))

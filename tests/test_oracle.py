@@ -1,0 +1,233 @@
+"""CPU checks that pin the oracle (oracle/flct_oracle.c) before it is
+trusted as the parity reference for the GPU path."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import f32, smooth_pair
+
+
+# ---------------------------------------------------------------- fixtures
+def test_flct_reproduces_reference_fixture(golden):
+    """flct hashgauss.dat testvels.dat 1 1 5 -k 0.5 (README.md:207-208) ->
+    tests/golden/testgaussvel.dat, at the file's float32 precision."""
+    vx, vy, vm = O.flct(golden["f1"], golden["f2"], 1.0, 1.0, 5.0, kr=0.5)
+    assert np.array_equal(vm, golden["vm"])
+    for a, r in ((vx, golden["vx"]), (vy, golden["vy"])):
+        same = np.sum(f32(a) == r)
+        # values that differ are ~1e-8 in size and differ by a float32 ulp
+        assert same >= 0.998 * r.size
+        assert np.max(np.abs(f32(a) - r)) < 1e-11
+
+
+def test_warp_reproduces_reference_fixture(golden):
+    """hashgauss f2 = shift_frac2d.pro(f1, deltax, deltay) (README.md:257-261),
+    stored as float32: warp_frac2d reproduces it to the fixture precision."""
+    w = O.warp_frac2d(golden["f1"], golden["dx"], golden["dy"])
+    assert np.max(np.abs(w - golden["f2"])) < 5e-7
+    assert np.sqrt(np.mean((w - golden["f2"]) ** 2)) < 1e-7
+
+
+# ---------------------------------------------------------------- closed forms
+def test_make_freq_closed_form():
+    # flct.c:1651-1692: 0..n/2 (Nyquist positive) then negatives; odd n has +-n/2
+    assert list(O.make_freq(8)) == [0, 1, 2, 3, 4, -3, -2, -1]
+    assert list(O.make_freq(5)) == [0, 1, 2, -2, -1]
+    assert list(O.make_freq(1)) == [0]
+    for n in range(2, 40):
+        k = O.make_freq(n)
+        ref = np.fft.fftfreq(n, 1.0 / n)
+        if n % 2 == 0:
+            ref[n // 2] = n // 2
+        assert np.array_equal(k, ref)
+
+
+APPENDIX_B = {4: 16, 5: 20, 6: 24, 7: 28, 8: 32, 9: 36, 10: 40, 11: 44, 12: 48, 13: 52,
+              14: 60, 15: 64, 16: 68, 17: 72, 18: 76, 19: 80, 20: 84}
+
+
+def test_box_size_table():
+    # flct.c:567-580: N = 2 * min((nt/2)*2, ndmin), nt = (int)((int)sigma * sqrt(ln 100))
+    for sig, n in APPENDIX_B.items():
+        assert 2 * O.box_nsize(sig, 4096, 4096) == n
+        assert 2 * O.box_nsize(sig + 0.7, 4096, 4096) == n  # sigma truncated first
+    assert 2 * O.box_nsize(15, 90, 4096) == 2 * ((90 // 3) // 2) * 2  # ndmin clamp
+    assert O.box_nsize(15, 5, 5) == -1  # "illegal box size"
+
+
+def test_fft_against_numpy_and_brute_force():
+    rng = np.random.default_rng(0)
+    for shape in [(16, 16), (20, 28), (64, 64), (84, 76), (9, 7), (101, 201)]:
+        a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        ref = np.fft.fft2(a)
+        got = O.fft2(a, -1)
+        assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
+        back = O.fft2(got, +1) / a.size
+        assert np.max(np.abs(back - a)) < 1e-13
+    # brute-force DFT on a tiny odd/prime size
+    a = rng.standard_normal((5, 7)) + 1j * rng.standard_normal((5, 7))
+    i, k = np.arange(5), np.arange(7)
+    D1 = np.exp(-2j * np.pi * np.outer(i, i) / 5)
+    D2 = np.exp(-2j * np.pi * np.outer(k, k) / 7)
+    assert np.max(np.abs(O.fft2(a, -1) - D1 @ a @ D2.T)) < 1e-12
+
+
+def test_peak_fit_on_sampled_quadratic():
+    """The 3x3 Taylor fit (flct.c:1536-1556) returns the exact vertex of a
+    sampled 2-D quadratic with a cross term."""
+    n = 16
+    x0, y0 = 8.3, 7.6
+    i, j = np.mgrid[0:n, 0:n].astype(float)
+    dx, dy = i - x0, j - y0
+    surf = 100.0 - (1.3 * dx * dx + 0.4 * dx * dy + 0.9 * dy * dy)
+    sx, sy = O.peak_fit(surf)
+    assert abs(sx - (x0 - n // 2)) < 1e-12 and abs(sy - (y0 - n // 2)) < 1e-12
+
+
+def test_peak_edge_and_zero_rules():
+    n = 12
+    s = np.zeros((n, n))
+    assert O.peak_fit(s) == (0.0, 0.0)  # max == 0 -> centre, no fit
+    s[0, 5] = 1.0  # edge peak: integer shift, no fit
+    assert O.peak_fit(s) == (0.0 - n // 2, 5.0 - n // 2)
+    s[:] = 0.0
+    s[3, 4] = s[7, 2] = 2.0  # tie: the first in row-major order wins
+    sx, sy = O.peak_fit(s)
+    assert np.floor(sx + n // 2 + 0.5) == 3 and np.floor(sy + n // 2 + 0.5) == 4
+
+
+def test_interpcc2d_reproduces_quadratics_and_border_rule():
+    """Keys cubic convolution (a = -0.5) is exact for quadratics in the
+    interior; the one-cell border uses f[2] - 3 f[1] + 3 f[0] (flct.c:2294)."""
+    nx, ny = 9, 11
+    i, j = np.mgrid[0:nx, 0:ny].astype(float)
+    q = 0.5 + 0.3 * i - 0.2 * j + 0.05 * i * i + 0.02 * i * j - 0.03 * j * j
+    xw = np.linspace(1.0, nx - 2.0, 13)
+    yw = np.linspace(1.0, ny - 2.0, 17)
+    out = O.interpcc2d(q, -999999.0, xw, yw)
+    X, Y = np.meshgrid(xw, yw, indexing="ij")
+    exact = 0.5 + 0.3 * X - 0.2 * Y + 0.05 * X * X + 0.02 * X * Y - 0.03 * Y * Y
+    assert np.max(np.abs(out - exact)) < 1e-12
+    # out of range -> xmiss
+    o2 = O.interpcc2d(q, -7.0, np.array([-0.1, nx - 1 + 0.1]), np.array([1.0]))
+    assert np.all(o2 == -7.0)
+    # border extrapolation by hand at x = 0.5 (needs cdata row 0)
+    f = np.arange(nx * ny, dtype=float).reshape(nx, ny) ** 1.5
+    x, y = 0.5, 4.0
+    got = O.interpcc2d(f, -999999.0, np.array([x]), np.array([y]))[0, 0]
+
+    def keys(t):
+        t = abs(t)
+        return (((-0.5 * t + 2.5) * t) - 4.0) * t + 2.0 if t >= 1 else (1.5 * t - 2.5) * t * t + 1.0
+
+    col = lambda r: f[r, :] if r >= 0 else f[2, :] - 3.0 * f[1, :] + 3.0 * f[0, :]
+    exp = sum(col(ii)[jj] * keys(x - ii) * keys(y - jj) for ii in (-1, 0, 1, 2) for jj in (3, 4, 5, 6))
+    assert abs(got - exp) < 1e-9 * abs(exp)
+    # a missing value in the 4x4 stencil poisons the result
+    g = q.copy()
+    g[4, 5] = -999999.0
+    o3 = O.interpcc2d(g, -999999.0, np.array([4.2, 1.0]), np.array([5.3, 1.0]))
+    assert o3[0, 0] == -999999.0 and o3[1, 1] != -999999.0
+
+
+# ---------------------------------------------------------------- invariants
+def test_identical_and_negated_images_give_zero_velocity():
+    rng = np.random.default_rng(3)
+    a, _ = smooth_pair(rng, 40, 44)
+    for b in (a, -a):
+        vx, vy, vm = O.flct(a, b, 1.0, 1.0, 4.0)
+        assert np.all(vm == 1.0)
+        assert np.max(np.abs(vx)) < 1e-10 and np.max(np.abs(vy)) < 1e-10
+
+
+def test_swapped_images_reverse_velocity():
+    rng = np.random.default_rng(4)
+    a, b = smooth_pair(rng, 48, 48)
+    v1 = O.flct(a, b, 1.0, 1.0, 5.0)
+    v2 = O.flct(b, a, 1.0, 1.0, 5.0)
+    # interior peaks: the fit is symmetric under f1 <-> f2
+    sel = (np.abs(v1[0]) < 3) & (np.abs(v1[1]) < 3)
+    assert np.max(np.abs(v1[0][sel] + v2[0][sel])) < 1e-9
+    assert np.max(np.abs(v1[1][sel] + v2[1][sel])) < 1e-9
+
+
+def test_integer_circular_shift_is_recovered():
+    rng = np.random.default_rng(5)
+    from scipy.ndimage import gaussian_filter
+    g = gaussian_filter(rng.standard_normal((24, 24)), 1.5, mode="wrap")
+    h = np.roll(np.roll(g, 2, axis=0), -3, axis=1)
+    sx, sy, _ = O.cross_cor(g, h)
+    assert abs(sx - 2.0) < 0.3 and abs(sy + 3.0) < 0.3
+
+
+def test_zero_images_hit_the_max_zero_branch():
+    z = np.zeros((40, 40))
+    vx, vy, vm = O.flct(z, z, 1.0, 1.0, 4.0)
+    assert np.all(vx == 0) and np.all(vy == 0) and np.all(vm == 1)
+
+
+def test_uniform_shift_recovers_known_velocity():
+    """Image 2 moved by (+0.3, -0.2) px in IDL (x, y): vx > 0, vy < 0 with
+    FLCT's known bias toward zero (Fisher & Welsch 2008, p. 379-380)."""
+    from pyflct_amd import synth
+    f1, f2 = synth.cfg1()
+    vx, vy, vm = O.flct(f1, f2, 1.0, 1.0, 8.0)
+    core = (slice(32, 96), slice(32, 96))
+    assert 0.1 < np.median(vx[core]) < 0.3
+    assert -0.2 < np.median(vy[core]) < -0.05
+
+
+def test_bias_correction_reduces_bias():
+    """-b (window offset by -+d/2, no reference counterpart): on uniform
+    shifts of a smooth periodic field the median error shrinks."""
+    from scipy.ndimage import gaussian_filter
+    from pyflct_amd import synth
+    rng = np.random.default_rng(1)
+    a = gaussian_filter(rng.standard_normal((96, 96)), 2.0, mode="wrap")
+    c = (slice(24, 72), slice(24, 72))
+    for d in (0.5, 1.5, 2.5):
+        b = synth.shift_uniform(a, d, -0.5 * d)
+        e = []
+        for bias in (False, True):
+            vx, vy, _ = O.flct(a, b, 1.0, 1.0, 6.0, bias=bias)
+            e.append(abs(np.median(vx[c]) - d) + abs(np.median(vy[c]) + 0.5 * d))
+        assert e[1] < 0.75 * e[0]
+    # zero motion stays zero
+    vx, vy, _ = O.flct(a, a, 1.0, 1.0, 6.0, bias=True)
+    assert np.max(np.abs(vx)) < 1e-10 and np.max(np.abs(vy)) < 1e-10
+
+
+def test_deltat_deltas_scaling():
+    rng = np.random.default_rng(6)
+    a, b = smooth_pair(rng, 40, 40)
+    v1 = O.flct(a, b, 1.0, 1.0, 4.0)
+    v2 = O.flct(a, b, 2.0, 3.0, 4.0)
+    assert np.allclose(v2[0], v1[0] * 1.5, rtol=0, atol=1e-12)
+    assert np.allclose(v2[1], v1[1] * 1.5, rtol=0, atol=1e-12)
+
+
+def test_threshold_relative_and_absolute_sets():
+    rng = np.random.default_rng(7)
+    a, b = smooth_pair(rng, 36, 36)
+    fmax = max(np.max(np.abs(a)), np.max(np.abs(b)))
+    below = 0.5 * np.abs(a + b) < 0.3 * fmax
+    _, _, vm = O.flct(a, b, 1.0, 1.0, 4.0, thresh=0.3)
+    assert np.array_equal(vm == 0, below)
+    _, _, vm2 = O.flct(a, b, 1.0, 1.0, 4.0, thresh=0.3, thresh_abs=True)
+    assert np.array_equal(vm2 == 0, 0.5 * np.abs(a + b) < 0.3)
+    assert O.resolve_thresh(a, b, 0.3) == 0.3 * fmax
+    assert O.resolve_thresh(a, b, 1.5) == 1.5  # >= 1 is absolute
+
+
+def test_skip_lattice_and_fill_mask():
+    rng = np.random.default_rng(8)
+    a, b = smooth_pair(rng, 41, 37)
+    for skip, p, q in [(3, 0, 0), (4, 1, 2), (5, -2, -1)]:
+        _, _, vm = O.flct(a, b, 1.0, 1.0, 4.0, skip=skip, poff=p, qoff=q)
+        pp, qq = (skip - abs(p) if p < 0 else p), (skip - abs(q) if q < 0 else q)
+        i, j = np.mgrid[0:41, 0:37]
+        lattice = ((i - qq) % skip == 0) & ((j - pp) % skip == 0)
+        assert np.array_equal(vm == 1, lattice)
+        _, _, vmi = O.flct(a, b, 1.0, 1.0, 4.0, skip=skip, poff=p, qoff=q, interp=True)
+        assert np.all(vmi[lattice] == 1)
+        assert set(np.unique(vmi[~lattice])) <= {0.0, 0.5}
