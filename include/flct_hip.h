@@ -100,6 +100,12 @@ int flct_hip_track_device(flct_hip_ctx *ctx, const double *d_f1, const double *d
 int flct_hip_last_stats(flct_hip_ctx *ctx, long long *tracked, double *kernel_ms,
                         double *total_ms);
 
+/* Number of pixels the last FLCT_HIP_SPLIT_BF16 call re-evaluated in FP64
+ * because the BF16 correlation surface could not settle them (a second cell
+ * within 1e-4 of the maximum, or an ill-conditioned peak fit); 0 for the
+ * other forms.  No counterpart in the reference, which has one precision. */
+int flct_hip_last_refined(flct_hip_ctx *ctx, long long *refined);
+
 /*
  * flct_hip_warp replaces warp_frac2d (warp.c:883-1011): out(i,j) is the
  * image's Fourier series evaluated at (i - shiftx, j - shifty), where with

@@ -41,6 +41,7 @@ EXPORTS = (
     "flct_hip_track",
     "flct_hip_track_device",
     "flct_hip_last_stats",
+    "flct_hip_last_refined",
     "flct_hip_warp",
     "flct_hip_warp_device",
     "flct_hip_shift",

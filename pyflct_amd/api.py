@@ -160,7 +160,10 @@ class Context:
         t = ctypes.c_double()
         check(_lib.lib().flct_hip_last_stats(self._h, ctypes.byref(n), ctypes.byref(k),
                                              ctypes.byref(t)))
-        return {"tracked": n.value, "kernel_ms": k.value, "total_ms": t.value}
+        rf = ctypes.c_longlong()
+        check(_lib.lib().flct_hip_last_refined(self._h, ctypes.byref(rf)))
+        return {"tracked": n.value, "kernel_ms": k.value, "total_ms": t.value,
+                "refined": rf.value}
 
     # ------------------------------------------------------------ warp
     def warp(self, arr, delx, dely, transp=True):

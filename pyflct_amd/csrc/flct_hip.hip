@@ -32,8 +32,9 @@ struct flct_hip_ctx {
   size_t lds_per_block = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   DevBuf f1, f2, vx, vy, vm, list, count, gw, filt, absmax, wtmp, wF, wtw1, wtw2, w3, w4, w5,
-      hs1, hs2, mean1, mean2;
+      hs1, hs2, mean1, mean2, flags, fcount;
   long long last_tracked = 0;
+  long long refined = 0;  // pixels re-evaluated in FP64 by the last split-BF16 call
   double last_kernel_ms = 0.;
   double last_total_ms = 0.;
 };
@@ -110,7 +111,8 @@ extern "C" int flct_hip_destroy(flct_hip_ctx* c) {
   (void)hipSetDevice(c->device);
   DevBuf* bufs[] = {&c->f1, &c->f2, &c->vx, &c->vy, &c->vm, &c->list, &c->count, &c->gw,
                     &c->filt, &c->absmax, &c->wtmp, &c->wF, &c->wtw1, &c->wtw2, &c->w3,
-                    &c->w4, &c->w5, &c->hs1, &c->hs2, &c->mean1, &c->mean2};
+                    &c->w4, &c->w5, &c->hs1, &c->hs2, &c->mean1, &c->mean2, &c->flags,
+                    &c->fcount};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (auto& e : c->ev)
@@ -126,6 +128,12 @@ extern "C" int flct_hip_last_stats(flct_hip_ctx* c, long long* tracked, double* 
   if (tracked) *tracked = c->last_tracked;
   if (kernel_ms) *kernel_ms = c->last_kernel_ms;
   if (total_ms) *total_ms = c->last_total_ms;
+  return FLCT_HIP_OK;
+}
+
+extern "C" int flct_hip_last_refined(flct_hip_ctx* c, long long* refined) {
+  if (!c || !refined) return FLCT_HIP_E_ARG;
+  *refined = c->refined;
   return FLCT_HIP_OK;
 }
 
@@ -192,7 +200,9 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
   Resolved r;
   int rc = resolve(o, nx, ny, &r);
   if (rc) return rc;
-  if (o->precision != FLCT_HIP_FP64 && o->precision != FLCT_HIP_FP32) return FLCT_HIP_E_UNSUPPORTED;
+  if (o->precision != FLCT_HIP_FP64 && o->precision != FLCT_HIP_FP32 &&
+      o->precision != FLCT_HIP_SPLIT_BF16)
+    return FLCT_HIP_E_UNSUPPORTED;
   const int N = r.N, M = N / 4;
   if (N % 4 != 0 || M < 1 || M > kTrackMaxM) return FLCT_HIP_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->device));
@@ -325,10 +335,48 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
   ta.vx = vx;
   ta.vy = vy;
   ta.vm = vm;
+  ta.absmax = nullptr;
+  ta.flag_list = nullptr;
+  ta.flag_count = nullptr;
+  c->refined = 0;
+  if (o->precision == FLCT_HIP_SPLIT_BF16 && nwork > 0) {
+    // pixels the BF16 surface cannot settle are listed for FP64 re-evaluation
+    if ((rc = ensure(c->flags, npix * sizeof(int))) || (rc = ensure(c->fcount, sizeof(unsigned int))))
+      return rc;
+    HIPCHK(hipMemsetAsync(c->fcount.p, 0, sizeof(unsigned int), s));
+    ta.flag_list = (int*)c->flags.p;
+    ta.flag_count = (unsigned int*)c->fcount.p;
+    // image maxima for the power-of-two input scaling of the BF16 form
+    if ((rc = ensure(c->absmax, 2 * sizeof(unsigned long long)))) return rc;
+    auto* am = (unsigned long long*)c->absmax.p;
+    HIPCHK(hipMemsetAsync(am, 0, 2 * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, f1, npix, am);
+    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, f2, npix, am + 1);
+    HIPCHK(hipGetLastError());
+    ta.absmax = am;
+  }
   HIPCHK(hipEventRecord(c->ev[1], s));
   if (nwork > 0) {
     LaunchCtx lc{c->stream, c->num_cu, c->lds_per_block};
-    if (o->precision == FLCT_HIP_FP32)
+    if (o->precision == FLCT_HIP_SPLIT_BF16) {
+      rc = dispatch_track_bf16(N, lc, ta);
+      if (!rc) {
+        unsigned int nflag = 0;
+        HIPCHK(hipMemcpyAsync(&nflag, c->fcount.p, sizeof(nflag), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        c->refined = nflag;
+        if (nflag > 0) {
+          TrackArgs tb = ta;
+          tb.list = (const int*)c->flags.p;
+          tb.nwork = nflag;
+          tb.absmax = nullptr;
+          tb.flag_list = nullptr;
+          tb.flag_count = nullptr;
+          rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, tb) : dispatch_track_f64_hi(M, lc, tb);
+        }
+      }
+    }
+    else if (o->precision == FLCT_HIP_FP32)
       rc = (M <= 12) ? dispatch_track_f32_lo(M, lc, ta) : dispatch_track_f32_hi(M, lc, ta);
     else
       rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, ta) : dispatch_track_f64_hi(M, lc, ta);

@@ -41,6 +41,9 @@ struct TrackArgs {
   const double* filt;   // (N/2+1) x N filter values, or nullptr
   const int* list;      // tracked pixel list, or nullptr for all pixels in order
   long long nwork;
+  const unsigned long long* absmax;  // max |f1|, max |f2| as double bits, or nullptr
+  int* flag_list;                    // pixels to re-evaluate in FP64, or nullptr
+  unsigned int* flag_count;
   int bias;
   double* vx;
   double* vy;
@@ -52,9 +55,12 @@ template <int M, class T>
 struct TrackCfg {
   static constexpr int N = 4 * M;
   static constexpr int kSlotThreads = 2 * N;
-  static constexpr int kRow = N + 1;  // padded LDS row (complex elements)
-  static constexpr int kZBytes = N * kRow * int(sizeof(cplx<T>));
-  static constexpr int kSlotBytes = ((kZBytes + 4 * N * 8 + 9 * 8 + 32) + 15) / 16 * 16;
+  // LDS row stride (complex elements), = 4 mod 8: the four columns a quad
+  // reads from one row and the next quad's four from the next row fall in
+  // different bank groups
+  static constexpr int kRow = (N % 8 == 4) ? N + 8 : N + 4;
+  static constexpr int kZBytes = (N * kRow + 8) * int(sizeof(cplx<T>));
+  static constexpr int kSlotBytes = ((kZBytes + 4 * N * 8 + 9 * 8 + 48) + 15) / 16 * 16;
   static constexpr int waste(int p) {
     int t = p * kSlotThreads;
     int r = ((t + 63) / 64) * 64;
@@ -73,6 +79,25 @@ struct TrackCfg {
   static constexpr int kThreads = ((P * kSlotThreads + 63) / 64) * 64;
   static constexpr int kLdsBytes = P * kSlotBytes + (2 * N) * 8;
 };
+
+// LDS index of spectrum element (k, c).  Rows of output block p = k / M
+// are offset by delta(p) elements so that the four lanes of a quad, which
+// write rows M*p + s for p = 0..3 of one column, hit four bank groups.
+template <int M>
+__device__ __forceinline__ int zidx(int k, int c) {
+  constexpr int N = 4 * M;
+  constexpr int R = (N % 8 == 4) ? N + 8 : N + 4;
+  const int p = k / M;
+  const int delta = (M % 2 == 0) ? 2 * p : ((6 * p) & 7);
+  return k * R + c + delta;
+}
+
+// column at which a quad parks blocked element l (lane brev2(l / M), slot
+// l % M) of a private row: lane-strided, so the quad's stores are contiguous
+template <int M>
+__device__ __forceinline__ int park_col(int l) {
+  return 4 * (l % M) + brev2(l / M);
+}
 
 __device__ __forceinline__ unsigned long long dbits(double v) {
   return (unsigned long long)__double_as_longlong(v);
@@ -104,7 +129,6 @@ template <int M, class T>
 __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs a) {
   using Cfg = TrackCfg<M, T>;
   constexpr int N = 4 * M;
-  constexpr int R = Cfg::kRow;
   extern __shared__ __align__(16) unsigned char lds[];
   const int tid = threadIdx.x;
   const int slot = tid / Cfg::kSlotThreads;
@@ -122,26 +146,63 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
   double* nb = wy2 + N;
   double* res = nb + 9;
   unsigned long long* amax = reinterpret_cast<unsigned long long*>(res + 2);
+  unsigned long long* gmx = amax + 2;  // max |g| of image 1 and 2 (bits)
   unsigned int* aidx = reinterpret_cast<unsigned int*>(amax + 1);
   __syncthreads();
 
   const int nx = a.nx, ny = a.ny, nsize = a.nsize;
   const int ns = ny - N + 1;
   const T norm = T((1. / ((double)N * N)) * (1. / ((double)N * N)));
-  const long long nslots = (long long)gridDim.x * Cfg::P;
-  const long long first = (long long)blockIdx.x * Cfg::P + slot;
-  const long long iters = (a.nwork + nslots - 1) / nslots;
   const int npass = a.bias ? 2 : 1;
+  // Work distribution.  List mode: slots stride through the list.  All
+  // pixels: blocks are dealt to the 8 XCDs round robin, so XCD x = block % 8
+  // takes rows [x nx/8, (x+1) nx/8) and walks them in strips of kStrip
+  // columns, row by row inside a strip; the box rows it reads then stay in
+  // its L2.
+  constexpr int kXcd = 8, kStrip = 512;
+  const bool dense = (a.list == nullptr);
+  long long first, stride, count;
+  int r0 = 0, rows_b = 0;
+  if (dense) {
+    const int nband = (int)gridDim.x < kXcd ? (int)gridDim.x : kXcd;
+    const int x = blockIdx.x % nband;
+    const long long nbx = (gridDim.x - x + nband - 1) / nband;
+    r0 = (int)((long long)nx * x / nband);
+    rows_b = (int)((long long)nx * (x + 1) / nband) - r0;
+    first = (long long)(blockIdx.x / nband) * Cfg::P + slot;
+    stride = nbx * Cfg::P;
+    count = (long long)rows_b * ny;
+  } else {
+    first = (long long)blockIdx.x * Cfg::P + slot;
+    stride = (long long)gridDim.x * Cfg::P;
+    count = a.nwork;
+  }
+  const long long iters = (count + stride - 1) / stride;
+  const int full_strips = ny / kStrip;
 
   for (long long it = 0; it < iters; ++it) {
-    const long long w = first + it * nslots;
-    const bool active = in_slot && w < a.nwork;
+    const long long w = first + it * stride;
+    const bool active = in_slot && w < count;
     int i = 0, j = 0, imin = 0, jmin = 0;
     double f1bar = 0., f2bar = 0.;
     if (active) {
-      const long long p = a.list ? (long long)a.list[w] : w;
-      i = (int)(p / ny);
-      j = (int)(p - (long long)i * ny);
+      if (dense) {
+        const long long sw = (long long)rows_b * kStrip;
+        const int sidx = (int)(w / sw);
+        int width = kStrip;
+        long long t = w - (long long)sidx * sw;
+        if (sidx >= full_strips) {
+          width = ny - full_strips * kStrip;
+          t = w - (long long)full_strips * sw;
+        }
+        const int strip = sidx < full_strips ? sidx : full_strips;
+        i = r0 + (int)(t / width);
+        j = strip * kStrip + (int)(t % width);
+      } else {
+        const long long p = (long long)a.list[w];
+        i = (int)(p / ny);
+        j = (int)(p - (long long)i * ny);
+      }
       // box clamp, flct.c:773-786
       const int imin0 = (0 > (i - (nsize - 1))) ? 0 : i - (nsize - 1);
       const int imax0 = ((nx - 1) < (i + nsize)) ? nx - 1 : i + nsize;
@@ -185,28 +246,83 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
         if (stid == 0) {
           *amax = 0ull;
           *aidx = 0xffffffffu;
+          gmx[0] = 0ull;
+          gmx[1] = 0ull;
         }
       }
       __syncthreads();
-      // ------------- phase 1: column transforms, one column at a time
-      if (active) {
-        const int l = fresh(lane), quad = fresh(qd);
-        const double* p1 = a.f1 + (long long)imin * ny + jmin;
-        const double* p2 = a.f2 + (long long)imin * ny + jmin;
-        const int kb = M * brev2(l);
+      // ------------- phase 1: column transforms of the packed windowed
+      // images g1 + i g2.  In FP32 each image is first scaled by a power of
+      // two that brings its largest |g| to [1, 2): exact in floating point
+      // (every later value carries the same factor and the argmax and the
+      // fit are scale-invariant), and it keeps the two packed images at
+      // comparable magnitude so that a nearly empty box (e.g. one that only
+      // grazes the limb of a disk) is not lost in the other image's rounding.
+      // FP64 has the margin to pack unscaled.
+      if constexpr (sizeof(T) == 8) {
+        if (active) {
+          const int l = fresh(lane), quad = fresh(qd);
+          const double* p1 = a.f1 + (long long)imin * ny + jmin;
+          const double* p2 = a.f2 + (long long)imin * ny + jmin;
+          const int kb = M * brev2(l);
 #pragma unroll 1
-        for (int t = 0; t < 2; ++t) {
-          const int c = quad + t * (N / 2);
-          const double w1c = wy1[c], w2c = wy2[c];
-          cplx<T> z[M];
-          static_for<M>([&](auto Rr) {
-            const int ii = 4 * Rr + l;
-            const long long off = (long long)ii * ny + c;
-            const double x1 = p1[off], x2 = p2[off];
-            z[Rr] = {T((wx1[ii] * w1c) * (x1 - f1bar)), T((wx2[ii] * w2c) * (x2 - f2bar))};
+          for (int t = 0; t < 2; ++t) {
+            const int c = quad + t * (N / 2);
+            const double w1c = wy1[c], w2c = wy2[c];
+            cplx<T> z[M];
+            static_for<M>([&](auto Rr) {
+              const int ii = 4 * Rr + l;
+              const long long off = (long long)ii * ny + c;
+              z[Rr] = {T((wx1[ii] * w1c) * (p1[off] - f1bar)),
+                       T((wx2[ii] * w2c) * (p2[off] - f2bar))};
+            });
+            quad_dit<M, -1>(z, l);
+            static_for<M>([&](auto S) { Z[zidx<M>(kb + S, c)] = z[S]; });
+          }
+        }
+      } else {
+        double ga[2][M], gb[2][M];
+        if (active) {
+          const int l = fresh(lane), quad = fresh(qd);
+          const double* p1 = a.f1 + (long long)imin * ny + jmin;
+          const double* p2 = a.f2 + (long long)imin * ny + jmin;
+          double m1 = 0., m2 = 0.;
+          static_for<2>([&](auto Tc) {
+            const int c = quad + Tc * (N / 2);
+            const double w1c = wy1[c], w2c = wy2[c];
+            static_for<M>([&](auto Rr) {
+              const int ii = 4 * Rr + l;
+              const long long off = (long long)ii * ny + c;
+              ga[Tc][Rr] = (wx1[ii] * w1c) * (p1[off] - f1bar);
+              gb[Tc][Rr] = (wx2[ii] * w2c) * (p2[off] - f2bar);
+              m1 = fmax(m1, fabs(ga[Tc][Rr]));
+              m2 = fmax(m2, fabs(gb[Tc][Rr]));
+            });
           });
-          quad_dit<M, -1>(z, l);
-          static_for<M>([&](auto S) { Z[(kb + S) * R + c] = z[S]; });
+          m1 = fmax(m1, dpp<kQuadXor1>(m1));
+          m2 = fmax(m2, dpp<kQuadXor1>(m2));
+          m1 = fmax(m1, dpp<kQuadXor2>(m1));
+          m2 = fmax(m2, dpp<kQuadXor2>(m2));
+          if (l == 0) {
+            atomicMax(&gmx[0], dbits(m1));
+            atomicMax(&gmx[1], dbits(m2));
+          }
+        }
+        __syncthreads();
+        if (active) {
+          const int l = fresh(lane), quad = fresh(qd);
+          const double g1max = __longlong_as_double((long long)gmx[0]);
+          const double g2max = __longlong_as_double((long long)gmx[1]);
+          const double s1 = (g1max > 0.) ? ldexp(1.0, -ilogb(g1max)) : 1.0;
+          const double s2 = (g2max > 0.) ? ldexp(1.0, -ilogb(g2max)) : 1.0;
+          const int kb = M * brev2(l);
+          static_for<2>([&](auto Tc) {
+            const int c = quad + Tc * (N / 2);
+            cplx<T> z[M];
+            static_for<M>([&](auto Rr) { z[Rr] = {T(ga[Tc][Rr] * s1), T(gb[Tc][Rr] * s2)}; });
+            quad_dit<M, -1>(z, l);
+            static_for<M>([&](auto S) { Z[zidx<M>(kb + S, c)] = z[S]; });
+          });
         }
       }
       __syncthreads();
@@ -220,7 +336,7 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
           cplx<T> u1[M], u2[M];
           static_for<M>([&](auto Rr) {
             const int jj = 4 * Rr + l;
-            const cplx<T> zk = Z[k * R + jj], zm = Z[km * R + jj];
+            const cplx<T> zk = Z[zidx<M>(k, jj)], zm = Z[zidx<M>(km, jj)];
             // A = (Zk + conj(Zm))/2, B = (Zk - conj(Zm))/(2i)
             u1[Rr] = {T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
             u2[Rr] = {T(0.5) * (zk.im + zm.im), T(-0.5) * (zk.re - zm.re)};
@@ -228,12 +344,12 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
           quad_dit<M, -1>(u1, l);
           // park A's spectrum in the mirror row, which only this quad reads
           wave_lds_order();
-          static_for<M>([&](auto S) { Z[km * R + lb + S] = u1[S]; });
+          static_for<M>([&](auto S) { Z[zidx<M>(km, 4 * S + l)] = u1[S]; });
           wave_lds_order();
           quad_dit<M, -1>(u2, l);
           wave_lds_order();
           static_for<M>([&](auto S) {
-            cplx<T> A = Z[km * R + lb + S], B = u2[S];
+            cplx<T> A = Z[zidx<M>(km, 4 * S + l)], B = u2[S];
             if (a.filt) {
               const T f = T(a.filt[k * N + lb + S]);
               A = {A.re * f, A.im * f};
@@ -242,30 +358,32 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
             u2[S] = cross_power(A, B, norm);
           });
           quad_dif<M, +1>(u2, l);
-          static_for<M>([&](auto Rr) { Z[k * R + 4 * Rr + l] = u2[Rr]; });
+          static_for<M>([&](auto Rr) { Z[zidx<M>(k, 4 * Rr + l)] = u2[Rr]; });
         } else {
           // rows k = 0 and k = N/2 of A and B are real along jj: one packed
           // transform per image, parked in rows N/2 (A) and 0 (B)
           cplx<T> u[M], v[M];
           static_for<M>([&](auto Rr) {
             const int jj = 4 * Rr + l;
-            const cplx<T> z0 = Z[0 * R + jj], zh = Z[(N / 2) * R + jj];
+            const cplx<T> z0 = Z[zidx<M>(0, jj)], zh = Z[zidx<M>(N / 2, jj)];
             u[Rr] = {z0.re, zh.re};
             v[Rr] = {z0.im, zh.im};
           });
           quad_dit<M, -1>(u, l);
           wave_lds_order();
-          static_for<M>([&](auto S) { Z[(N / 2) * R + lb + S] = u[S]; });
+          static_for<M>([&](auto S) { Z[zidx<M>(N / 2, 4 * S + l)] = u[S]; });
           wave_lds_order();
           quad_dit<M, -1>(v, l);
           wave_lds_order();
-          static_for<M>([&](auto S) { Z[0 * R + lb + S] = v[S]; });
+          static_for<M>([&](auto S) { Z[zidx<M>(0, 4 * S + l)] = v[S]; });
           wave_lds_order();
           static_for<M>([&](auto S) {
             const int lx = lb + S;
             const int lm = (N - lx) % N;
-            const cplx<T> U = Z[(N / 2) * R + lx], Um = Z[(N / 2) * R + lm];
-            const cplx<T> V = Z[0 * R + lx], Vm = Z[0 * R + lm];
+            const cplx<T> U = Z[zidx<M>(N / 2, park_col<M>(lx))];
+            const cplx<T> Um = Z[zidx<M>(N / 2, park_col<M>(lm))];
+            const cplx<T> V = Z[zidx<M>(0, park_col<M>(lx))];
+            const cplx<T> Vm = Z[zidx<M>(0, park_col<M>(lm))];
             // row 0 = (U + conj(Um))/2, row N/2 = (U - conj(Um))/(2i)
             cplx<T> A0 = {T(0.5) * (U.re + Um.re), T(0.5) * (U.im - Um.im)};
             cplx<T> Ah = {T(0.5) * (U.im + Um.im), T(-0.5) * (U.re - Um.re)};
@@ -285,8 +403,8 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
           wave_lds_order();
           static_for<M>([&](auto Rr) {
             const int jj = 4 * Rr + l;
-            Z[0 * R + jj] = {u[Rr].re, T(0)};
-            Z[(N / 2) * R + jj] = {u[Rr].im, T(0)};
+            Z[zidx<M>(0, jj)] = {u[Rr].re, T(0)};
+            Z[zidx<M>(N / 2, jj)] = {u[Rr].im, T(0)};
           });
         }
       }
@@ -294,7 +412,7 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
       // ------------- phase 3: c2r along k, two columns per transform
       T va[M], vb[M];
       const int l = fresh(lane), quad = fresh(qd);
-      const int ca = 2 * quad, cb = 2 * quad + 1;
+      const int ca = quad, cb = quad + N / 2;
       const int ib = M * brev2(l);
       const int sa = (ca + N / 2) % N, sb = (cb + N / 2) % N;
       if (active) {
@@ -303,11 +421,11 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
           const int k = 4 * Rr + l;
           cplx<T> qa, qb;
           if (k <= N / 2) {
-            qa = Z[k * R + ca];
-            qb = Z[k * R + cb];
+            qa = Z[zidx<M>(k, ca)];
+            qb = Z[zidx<M>(k, cb)];
           } else {
-            qa = cconj(Z[(N - k) * R + ca]);
-            qb = cconj(Z[(N - k) * R + cb]);
+            qa = cconj(Z[zidx<M>(N - k, ca)]);
+            qb = cconj(Z[zidx<M>(N - k, cb)]);
           }
           z[Rr] = {qa.re - qb.im, qa.im + qb.re};
         });

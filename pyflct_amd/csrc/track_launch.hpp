@@ -41,5 +41,7 @@ int dispatch_track_f64_lo(int M, const LaunchCtx& c, const TrackArgs& a);
 int dispatch_track_f64_hi(int M, const LaunchCtx& c, const TrackArgs& a);
 int dispatch_track_f32_lo(int M, const LaunchCtx& c, const TrackArgs& a);
 int dispatch_track_f32_hi(int M, const LaunchCtx& c, const TrackArgs& a);
+// split-BF16 MFMA form, box size N = 4..96
+int dispatch_track_bf16(int N, const LaunchCtx& c, const TrackArgs& a);
 
 }  // namespace flct
