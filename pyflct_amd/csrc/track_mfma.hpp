@@ -27,6 +27,15 @@
 
 namespace flct {
 
+// Pixels whose result the split-BF16 surface cannot settle are listed for
+// re-evaluation in FP64: another cell within kTieTol of the maximum, or a
+// Taylor fit whose first-order response to relative errors kRelErr of the
+// nine values exceeds kFitTol pixels.  kRelErr is a few times the observed
+// relative error of the surface values.
+constexpr double kTieTol = 1e-4;
+constexpr double kRelErr = 1e-5;
+constexpr double kFitTol = 1e-2;
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -121,6 +130,8 @@ __global__ void __launch_bounds__((MfmaCfg<NP>::kThreads)) k_track_mfma(TrackArg
   double* res = nb + 9;
   unsigned long long* amax = reinterpret_cast<unsigned long long*>(res + 2);
   unsigned int* aidx = reinterpret_cast<unsigned int*>(amax + 1);
+  unsigned int* aeq = aidx + 1;   // cells equal to the maximum
+  unsigned int* asec = aidx + 2;  // largest value below the maximum (float bits)
 
   // D in LDS: rows r, columns c < NP, zero outside the N x N DFT
   for (int e = tid; e < NP * NP; e += blockDim.x) {
@@ -174,6 +185,7 @@ __global__ void __launch_bounds__((MfmaCfg<NP>::kThreads)) k_track_mfma(TrackArg
       f2bar = a.mean2[(long long)imin * ns + jmin];
     }
     double shiftx = 0., shifty = 0.;
+    bool flagged = false;  // meaningful in thread ptid == 0
     for (int pass = 0; pass < npass; ++pass) {
       // ---------------- windows
       if (active) {
@@ -204,6 +216,8 @@ __global__ void __launch_bounds__((MfmaCfg<NP>::kThreads)) k_track_mfma(TrackArg
         if (ptid == 0) {
           *amax = 0ull;
           *aidx = 0xffffffffu;
+          *aeq = 0u;
+          *asec = 0u;
         }
       }
       __syncthreads();
@@ -470,19 +484,35 @@ __global__ void __launch_bounds__((MfmaCfg<NP>::kThreads)) k_track_mfma(TrackArg
       const double gmax = __longlong_as_double((long long)*amax);
       if (active) {
         unsigned best = 0xffffffffu;
+        unsigned neq = 0u;
+        float sec = 0.f;
         static_for<HT>([&](auto MT) {
           static_for<TP>([&](auto NT) {
             static_for<4>([&](auto t) {
               const int ii = (h * HT + MT) * 16 + 4 * fg + t, jj = NT * 16 + fr;
-              if (ii < N && jj < N && (double)cc[MT][NT][int(t)] == gmax) {
-                const unsigned L = (unsigned)(((ii + N / 2) % N) * N + (jj + N / 2) % N);
-                best = min(best, L);
+              if (ii < N && jj < N) {
+                const float v = cc[MT][NT][int(t)];
+                if ((double)v == gmax) {
+                  const unsigned L = (unsigned)(((ii + N / 2) % N) * N + (jj + N / 2) % N);
+                  best = min(best, L);
+                  ++neq;
+                } else {
+                  sec = fmaxf(sec, v);
+                }
               }
             });
           });
         });
-        for (int o = 32; o > 0; o >>= 1) best = min(best, (unsigned)__shfl_xor((int)best, o));
-        if (lane == 0 && best != 0xffffffffu) atomicMin(aidx, best);
+        for (int o = 32; o > 0; o >>= 1) {
+          best = min(best, (unsigned)__shfl_xor((int)best, o));
+          neq += (unsigned)__shfl_xor((int)neq, o);
+          sec = fmaxf(sec, __shfl_xor(sec, o));
+        }
+        if (lane == 0) {
+          if (best != 0xffffffffu) atomicMin(aidx, best);
+          atomicAdd(aeq, neq);
+          atomicMax(asec, __float_as_uint(sec));
+        }
       }
       __syncthreads();
       const unsigned loc = *aidx;
@@ -520,7 +550,16 @@ __global__ void __launch_bounds__((MfmaCfg<NP>::kThreads)) k_track_mfma(TrackArg
           const double fxy = 0.25 * (nb[8] + nb[0] - nb[6] - nb[2]);
           shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
           shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
+          // first-order bound of the fit's response to relative errors
+          // kRelErr of the nine values
+          const double det = fxx * fyy - fxy * fxy;
+          const double hn = fabs(fxx) + fabs(fyy) + 2. * fabs(fxy);
+          const double bound = kRelErr * gmax * hn * (1. + 4. * (fabs(shiftsubx) + fabs(shiftsuby)));
+          if (!(bound <= kFitTol * fabs(det))) flagged = true;
         }
+        // another cell within the split-BF16 error of the maximum
+        if (nonzero && (*aeq > 1u || (double)__uint_as_float(*asec) >= gmax * (1. - kTieTol)))
+          flagged = true;
         res[0] = ((double)ixmax + shiftsubx) - (double)(N / 2);
         res[1] = ((double)iymax + shiftsuby) - (double)(N / 2);
       }
@@ -533,6 +572,7 @@ __global__ void __launch_bounds__((MfmaCfg<NP>::kThreads)) k_track_mfma(TrackArg
       a.vx[p] = shifty * a.deltinv * a.deltas;
       a.vy[p] = shiftx * a.deltinv * a.deltas;
       a.vm[p] = 1.;
+      if (flagged && a.flag_list) a.flag_list[atomicAdd(a.flag_count, 1u)] = (int)p;
     }
   }
 }

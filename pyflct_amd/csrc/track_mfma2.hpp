@@ -40,15 +40,6 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2v __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
-// Pixels whose result the split-BF16 surface cannot settle are listed for
-// re-evaluation in FP64: another cell within kTieTol of the maximum, or a
-// Taylor fit whose first-order response to relative errors kRelErr of the
-// nine values exceeds kFitTol pixels.  kRelErr is a few times the observed
-// relative error of the surface values.
-constexpr double kTieTol = 1e-4;
-constexpr double kRelErr = 1e-5;
-constexpr double kFitTol = 1e-2;
-
 template <int PIX>
 struct R2Cfg {
   static constexpr int RS = 40;              // bf16 row stride of 32-wide rows (80 B)
@@ -344,27 +335,39 @@ __global__ void __launch_bounds__(R2Cfg<PIX>::kThreads, 2) k_track_r2(TrackArgs 
       }
 
       CF4 S1p[2], S1m[2], S2p[2], S2m[2];
-      static_for<2>([&](auto X) {
-        // ---------------- G: wave kt writes the rows x = 2x' + kt, lane = column c
+      // ---------------- G: wave kt covers the rows x = 2x' + kt, lane = column c.
+      // The raw rows of image 2 are loaded while image 1 is in stage 1.
+      double fpre[32];
+      auto load_rows = [&](auto X) {
         if (active) {
           const double* f = (X == 0) ? a.f1 : a.f2;
-          const double fbar = (X == 0) ? f1bar : f2bar;
-          // this wave's row windows w(x), x = 16 it4 + 2t + kt, as 4 x 8
-          // floats in LDS (written and read by this wave only)
-          float* wxt = wxs + kt * 32;
-          if ((lane & 1) == kt && lane < 64) wxt[lane >> 1] = wxl[X];
-          wave_lds_fence();
           // rows are wave-uniform: one scalar base plus a 32-bit byte offset
           // that advances by two rows per element (columns clamped into the
-          // box, rows beyond it read row 0; padding is zeroed after the load)
+          // box, rows beyond it read row 0; padding is zeroed when built)
           const char* fb = reinterpret_cast<const char*>(f + ((long long)imin * ny + jmin));
           const int cl = lane < N ? lane : N - 1;
-          const bool cval = lane < N;
-          const float wy = wyl[X];
           // opaque per pixel, so the per-row offsets are not hoisted out of
           // the pixel loop as 32 live registers
           const unsigned step = (unsigned)fresh_s(16 * ny);
           unsigned boff = (unsigned)fresh(8 * (kt * ny + cl));
+          static_for<32>([&](auto e) {
+            const int x = 2 * e + kt;
+            const unsigned bo = (x < N) ? boff : 8u * (unsigned)cl;
+            fpre[int(e)] = *reinterpret_cast<const double*>(fb + bo);
+            boff += step;
+          });
+        }
+      };
+      auto build_g = [&](auto X) {
+        if (active) {
+          const double fbar = (X == 0) ? f1bar : f2bar;
+          // this wave's row windows w(x), x = 16 it4 + 2t + kt, as 4 x 8
+          // floats in LDS (written and read by this wave only)
+          float* wxt = wxs + kt * 32;
+          if ((lane & 1) == kt) wxt[lane >> 1] = wxl[X];
+          wave_lds_fence();
+          const bool cval = lane < N;
+          const float wy = wyl[X];
           static_for<4>([&](auto it4) {
             const float4 wa = *reinterpret_cast<const float4*>(wxt + 8 * it4);
             const float4 wb = *reinterpret_cast<const float4*>(wxt + 8 * it4 + 4);
@@ -372,12 +375,8 @@ __global__ void __launch_bounds__(R2Cfg<PIX>::kThreads, 2) k_track_r2(TrackArgs 
             float v[8];
             static_for<8>([&](auto t) {
               const int x = 16 * it4 + 2 * t + kt;
-              const bool xv = x < N;
-              const unsigned bo = xv ? boff : 8u * (unsigned)cl;
-              const double fv = *reinterpret_cast<const double*>(fb + bo);
-              const float gv = (float)(fv - fbar) * (wy * wxv[int(t)]);
-              v[int(t)] = (xv && cval) ? gv : 0.f;
-              boff += step;
+              const float gv = (float)(fpre[8 * it4 + t] - fbar) * (wy * wxv[int(t)]);
+              v[int(t)] = (x < N && cval) ? gv : 0.f;
             });
             u32x4 hv, lv;
             static_for<4>([&](auto d) {
@@ -390,7 +389,14 @@ __global__ void __launch_bounds__(R2Cfg<PIX>::kThreads, 2) k_track_r2(TrackArgs 
             *reinterpret_cast<u32x4*>(GB + (2 * kt + 1) * PL + lane * RS + 8 * it4) = lv;
           });
         }
-        __syncthreads();
+      };
+      static_for<2>([&](auto X) {
+        if constexpr (X == 0) {
+          load_rows(X);
+          build_g(X);
+        }
+        __syncthreads();  // G of image X is complete
+        if constexpr (X == 0) load_rows(std::integral_constant<int, 1>{});
         // ---------------- stage 1: T[c][k] for c = 0..63, this wave's k
         CF4 T[4];
         static_for<4>([&](auto MT) {
@@ -408,8 +414,12 @@ __global__ void __launch_bounds__(R2Cfg<PIX>::kThreads, 2) k_track_r2(TrackArgs 
             T[MT].im[int(t)] = k0 ? (Er[int(t)] - Or[int(t)]) : ti;
           });
         });
-        // G is read by both waves before it is overwritten
-        if (X == 0) __syncthreads();
+        // both waves have read G of image 1; build image 2 (its rows were
+        // loaded during stage 1) while image 1 goes on to stage 2
+        if constexpr (X == 0) {
+          __syncthreads();
+          build_g(std::integral_constant<int, 1>{});
+        }
         // B operands of stage 2: set q holds the rows c = 16 MT + 4g + q (+2)
         CFrag B[2];
         static_for<2>([&](auto q) {

@@ -1,0 +1,84 @@
+"""Split-BF16 MFMA form against the FP64 plain form (GPU).
+
+The precision form replaces the per-pixel FFTs by DFT-matrix contractions in
+BF16 hi/lo operands with FP32 accumulation; window, means, peak fit and bias
+stay FP64.  Its contract: the vm mask and the thresholded / skipped pixel set
+are bit-exact (they never depend on the correlation surface), and the RMS
+velocity error against the FP64 form is <= 1e-4 px.  Pixels whose BF16
+surface has a near tie or an ill-conditioned fit are recomputed in FP64, so
+the largest error is bounded too (5e-3 px here, observed ~1.2e-3).
+"""
+import numpy as np
+import pytest
+
+from pyflct_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+RMS_TOL = 1e-4   # px, RMS over all tracked pixels
+MAX_TOL = 5e-3   # px, any pixel
+
+# (n, sigma, options): box sizes 20 and 84 use the dense-DFT kernel, 36..64
+# the radix-2 kernel
+CASES = [
+    (192, 5.0, {}),                         # N = 20
+    (256, 9.0, {}),                         # N = 36
+    (256, 12.0, {"kr": 0.5}),               # N = 48, filter
+    (384, 15.0, {}),                        # N = 64
+    (256, 12.0, {"bias": True}),            # bias pass
+    (384, 15.0, {"thresh": 0.2, "skip": 3}),  # thresholded lattice
+    (256, 20.0, {}),                        # N = 84
+]
+
+
+def _errs(a, b):
+    e = np.hypot(b[0] - a[0], b[1] - a[1])
+    return float(np.sqrt(np.mean(e ** 2))), float(e.max())
+
+
+@pytest.mark.parametrize("n,sigma,kw", CASES, ids=[f"n{c[0]}-s{c[1]:g}-{'-'.join(c[2]) or 'plain'}"
+                                                    for c in CASES])
+def test_bf16_matches_fp64(ctx, n, sigma, kw):
+    f1, f2, _ = synth.cfg4(n=n)
+    ref = ctx.track(f1, f2, 1.0, 1.0, sigma, **kw)
+    st_ref = ctx.last_stats()
+    got = ctx.track(f1, f2, 1.0, 1.0, sigma, precision="split_bf16", **kw)
+    st = ctx.last_stats()
+    assert st["tracked"] == st_ref["tracked"]
+    assert np.array_equal(got[2], ref[2])
+    assert not np.isnan(got[0]).any() and not np.isnan(got[1]).any()
+    rms, mx = _errs(ref, got)
+    assert rms <= RMS_TOL, (rms, mx)
+    assert mx <= MAX_TOL, (rms, mx)
+
+
+def test_bf16_refinement_settles_outliers(ctx):
+    """On the full-disk field at 1024^2 the BF16 surfaces of about 0.1 % of
+    the pixels have near ties (e.g. peaks half-way between two cells) whose
+    fit flips between neighbours; those are recomputed in FP64."""
+    f1, f2, _ = synth.cfg4(n=1024)
+    ref = ctx.track(f1, f2, 1.0, 1.0, 15.0)
+    got = ctx.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
+    st = ctx.last_stats()
+    assert 0 < st["refined"] < 0.01 * st["tracked"]
+    rms, mx = _errs(ref, got)
+    assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
+
+
+def test_bf16_scale_invariance(ctx):
+    """Inputs are scaled by powers of two before the split; results must not
+    depend on the overall amplitude (tiny and huge images)."""
+    rng = np.random.default_rng(11)
+    from conftest import smooth_pair
+    a, b = smooth_pair(rng, 160, 176)
+    base = ctx.track(a, b, 1.0, 1.0, 12.0, precision="split_bf16")
+    for s in (2.0 ** -60, 2.0 ** 70):
+        got = ctx.track(a * s, b * s, 1.0, 1.0, 12.0, precision="split_bf16")
+        assert np.array_equal(got[0], base[0]) and np.array_equal(got[1], base[1])
+        assert np.array_equal(got[2], base[2])
+
+
+def test_bf16_other_forms_report_no_refinement(ctx):
+    f1, f2, _ = synth.cfg4(n=192)
+    ctx.track(f1, f2, 1.0, 1.0, 9.0)
+    assert ctx.last_stats()["refined"] == 0
