@@ -21,7 +21,7 @@ METRIC = "tracked px/s at 4096^2 sigma=15: split-BF16 MFMA vs FP64 plain; RMS ve
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
-PMC_FILE = os.path.join(ROOT, "profiles", "track_f64_4096_sigma15_traffic.json")
+PMC_FILE = os.path.join(ROOT, "profiles", "track_4096_sigma15_traffic.json")
 
 
 def w_fft(n):
@@ -29,6 +29,22 @@ def w_fft(n):
     FFTs (2.5 N^2 log2 N^2 each) plus window, mean, filter, product, |.|,
     argmax: 15 N^2 log2 N + 14 N^2."""
     return 15.0 * n * n * math.log2(n) + 14.0 * n * n
+
+
+def w_gemm(n):
+    """Contraction flops per tracked pixel of the split-BF16 form as dense
+    DFT-matrix products: forward 2 [4 N^3 + 8 N^2 (N/2+1)] (two real images),
+    inverse 8 N^2 (N/2+1) + 4 N^3, times 3 for hi*hi + hi*lo + lo*hi."""
+    h = n // 2 + 1
+    return 3.0 * (2 * (4 * n ** 3 + 8 * n * n * h) + (8 * n * n * h + 4 * n ** 3))
+
+
+def r2_mfma_flops(n, bias=False):
+    """MFMA flops the radix-2 split-BF16 kernel executes per tracked pixel
+    (boxes 36..64): per wave and pass 88 three-product groups of
+    16x16x32 BF16 MFMAs (stage 1: 32, stage 2: 32, inverse 1: 16, inverse 2:
+    8), two waves per pixel."""
+    return 88 * 3 * (2 * 16 * 16 * 32) * 2 * (2 if bias else 1)
 
 
 def log(*a):
@@ -95,11 +111,14 @@ def main():
             log(f"bench: form {form} not available: {e}")
             continue
         log(f"bench: timing {form}, {args.steps} steps")
-        kms = []
+        kms, mms, rms_ = [], [], []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             ctx.track_device(d["f1"], d["f2"], nx, ny, d["vx"], d["vy"], d["vm"], opts)
-            kms.append(ctx.last_stats()["kernel_ms"])
+            st = ctx.last_stats()
+            kms.append(st["kernel_ms"])
+            mms.append(st["main_ms"])
+            rms_.append(st["refine_ms"])
         t1 = time.perf_counter()
         st = ctx.last_stats()
         ms = (t1 - t0) * 1e3 / args.steps
@@ -107,7 +126,9 @@ def main():
         vy = ctx.download(np.empty_like(f1), d["vy"])
         results[form] = {"ms_per_step": ms, "tracked": st["tracked"],
                          "px_per_s": st["tracked"] / (ms * 1e-3),
-                         "kernel_ms": float(np.mean(kms)), "vx": vx, "vy": vy}
+                         "kernel_ms": float(np.mean(kms)), "main_ms": float(np.mean(mms)),
+                         "refine_ms": float(np.mean(rms_)), "refined": st["refined"],
+                         "vx": vx, "vy": vy}
         if form == "fp64":
             ref = (vx, vy)
         log(f"bench: {form}: {ms:.1f} ms/step, kernel {np.mean(kms):.1f} ms, "
@@ -138,24 +159,28 @@ def main():
     hr = results[head]
     dtype = {"fp64": "f64", "fp32": "f32", "split_bf16": "bf16x3+f32acc"}[head]
 
-    flops = w_fft(nbox) * hr["tracked"]
     if head == "fp64":
-        peak, bound = PEAK_FP64_TFLOPS, "mfma"
+        peak, per_px, kname = PEAK_FP64_TFLOPS, w_fft(nbox), f"k_track<{nbox // 4},double>"
     elif head == "fp32":
-        peak, bound = PEAK_FP32_TFLOPS, "mfma"
+        peak, per_px, kname = PEAK_FP32_TFLOPS, w_fft(nbox), f"k_track<{nbox // 4},float>"
     else:
-        peak, bound = PEAK_BF16_TFLOPS, "mfma"
-    achieved = flops / (hr["kernel_ms"] * 1e-3) / 1e12
+        peak, per_px = PEAK_BF16_TFLOPS, w_gemm(nbox)
+        kname = "k_track_r2<2>" if 36 <= nbox <= 64 else "k_track_mfma"
+    flops = per_px * hr["tracked"]
+    achieved = flops / (hr["main_ms"] * 1e-3) / 1e12
     traffic = None
     if os.path.exists(PMC_FILE):
         try:
             traffic = json.load(open(PMC_FILE)).get(head, {}).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
-    roof = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": traffic,
-            "kernel": f"k_track<{nbox // 4},{'double' if head == 'fp64' else 'float'}>",
-            "flops_per_px": w_fft(nbox), "kernel_ms": round(hr["kernel_ms"], 3)}
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": kname,
+            "flops_per_px": per_px, "kernel_ms": round(hr["main_ms"], 3)}
+    if head == "split_bf16" and 36 <= nbox <= 64:
+        ex = r2_mfma_flops(nbox) * hr["tracked"] / (hr["main_ms"] * 1e-3) / 1e12
+        roof["executed_mfma_flops_per_px"] = r2_mfma_flops(nbox)
+        roof["mfma_util"] = round(ex / peak, 4)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -181,7 +206,8 @@ def main():
                                f"({nbox}x{nbox} boxes), all pixels tracked",
                    "form": head, "box": nbox, "tracked_px": hr["tracked"]},
         "forms": {f: {"px_per_s": round(r["px_per_s"], 1), "ms_per_step": round(r["ms_per_step"], 3),
-                      "kernel_ms": round(r["kernel_ms"], 3),
+                      "kernel_ms": round(r["kernel_ms"], 3), "main_ms": round(r["main_ms"], 3),
+                      "refine_ms": round(r["refine_ms"], 3), "refined_px": r["refined"],
                       "rms_vel_err_vs_fp64": r["rms_vel_err_vs_fp64"],
                       "max_vel_err_vs_fp64": r["max_vel_err_vs_fp64"]}
                   for f, r in results.items()},

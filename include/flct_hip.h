@@ -106,6 +106,12 @@ int flct_hip_last_stats(flct_hip_ctx *ctx, long long *tracked, double *kernel_ms
  * other forms.  No counterpart in the reference, which has one precision. */
 int flct_hip_last_refined(flct_hip_ctx *ctx, long long *refined);
 
+/* Device time in milliseconds of the last track call's tracking kernel alone
+ * (main_ms) and of the FP64 re-evaluation of flagged pixels (refine_ms, 0
+ * except for FLCT_HIP_SPLIT_BF16); HIP events on the context's stream.  No
+ * counterpart in the reference. */
+int flct_hip_last_kernel_ms(flct_hip_ctx *ctx, double *main_ms, double *refine_ms);
+
 /*
  * flct_hip_warp replaces warp_frac2d (warp.c:883-1011): out(i,j) is the
  * image's Fourier series evaluated at (i - shiftx, j - shifty), where with

@@ -42,6 +42,7 @@ EXPORTS = (
     "flct_hip_track_device",
     "flct_hip_last_stats",
     "flct_hip_last_refined",
+    "flct_hip_last_kernel_ms",
     "flct_hip_warp",
     "flct_hip_warp_device",
     "flct_hip_shift",
@@ -101,6 +102,7 @@ def lib():
         L.flct_hip_track_device.argtypes = [vp, vp, vp, i, i, ctypes.POINTER(Opts), vp, vp, vp]
         L.flct_hip_last_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong),
                                           ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.flct_hip_last_kernel_ms.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d)]
         L.flct_hip_warp.argtypes = [vp, _dp, _dp, _dp, _dp, i, i, i, i]
         L.flct_hip_warp_device.argtypes = [vp, vp, vp, vp, vp, i, i, i]
         L.flct_hip_shift.argtypes = [vp, _dp, d, d, _dp, i, i, i, i]

@@ -162,8 +162,11 @@ class Context:
                                              ctypes.byref(t)))
         rf = ctypes.c_longlong()
         check(_lib.lib().flct_hip_last_refined(self._h, ctypes.byref(rf)))
+        mm = ctypes.c_double()
+        rm = ctypes.c_double()
+        check(_lib.lib().flct_hip_last_kernel_ms(self._h, ctypes.byref(mm), ctypes.byref(rm)))
         return {"tracked": n.value, "kernel_ms": k.value, "total_ms": t.value,
-                "refined": rf.value}
+                "refined": rf.value, "main_ms": mm.value, "refine_ms": rm.value}
 
     # ------------------------------------------------------------ warp
     def warp(self, arr, delx, dely, transp=True):

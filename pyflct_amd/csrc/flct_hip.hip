@@ -30,13 +30,15 @@ struct flct_hip_ctx {
   hipStream_t stream = nullptr;
   int num_cu = 0;
   size_t lds_per_block = 0;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   DevBuf f1, f2, vx, vy, vm, list, count, gw, filt, absmax, wtmp, wF, wtw1, wtw2, w3, w4, w5,
       hs1, hs2, mean1, mean2, flags, fcount;
   long long last_tracked = 0;
   long long refined = 0;  // pixels re-evaluated in FP64 by the last split-BF16 call
   double last_kernel_ms = 0.;
   double last_total_ms = 0.;
+  double last_main_ms = 0.;    // the tracking kernel alone
+  double last_refine_ms = 0.;  // FP64 re-evaluation of flagged pixels (split-BF16 form)
 };
 
 #define HIPCHK(x)                              \
@@ -134,6 +136,13 @@ extern "C" int flct_hip_last_stats(flct_hip_ctx* c, long long* tracked, double* 
 extern "C" int flct_hip_last_refined(flct_hip_ctx* c, long long* refined) {
   if (!c || !refined) return FLCT_HIP_E_ARG;
   *refined = c->refined;
+  return FLCT_HIP_OK;
+}
+
+extern "C" int flct_hip_last_kernel_ms(flct_hip_ctx* c, double* main_ms, double* refine_ms) {
+  if (!c) return FLCT_HIP_E_ARG;
+  if (main_ms) *main_ms = c->last_main_ms;
+  if (refine_ms) *refine_ms = c->last_refine_ms;
   return FLCT_HIP_OK;
 }
 
@@ -361,6 +370,7 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
     if (o->precision == FLCT_HIP_SPLIT_BF16) {
       rc = dispatch_track_bf16(N, lc, ta);
       if (!rc) {
+        HIPCHK(hipEventRecord(c->ev[4], s));
         unsigned int nflag = 0;
         HIPCHK(hipMemcpyAsync(&nflag, c->fcount.p, sizeof(nflag), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -415,6 +425,13 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
   c->last_tracked = nwork;
   c->last_kernel_ms = ev_ms(c->ev[1], c->ev[2]);
   c->last_total_ms = ev_ms(c->ev[0], c->ev[3]);
+  if (o->precision == FLCT_HIP_SPLIT_BF16 && nwork > 0) {
+    c->last_main_ms = ev_ms(c->ev[1], c->ev[4]);
+    c->last_refine_ms = ev_ms(c->ev[4], c->ev[2]);
+  } else {
+    c->last_main_ms = c->last_kernel_ms;
+    c->last_refine_ms = 0.;
+  }
   return FLCT_HIP_OK;
 }
 
