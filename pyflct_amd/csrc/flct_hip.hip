@@ -32,7 +32,10 @@ struct flct_hip_ctx {
   size_t lds_per_block = 0;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   DevBuf f1, f2, vx, vy, vm, list, count, gw, filt, absmax, wtmp, wF, wtw1, wtw2, w3, w4, w5,
-      hs1, hs2, mean1, mean2, flags, fcount;
+      hs1, hs2, mean1, mean2, flags, fcount, segtab, elist, ecount;
+  int seg_n = 0;          // box size of the split tables in segtab
+  double seg_sigma = 0.;  // window width of the split tables
+  int bf16_kernel = 0;    // FLCT_HIP_BF16_KERNEL: 0 automatic, 1 per-pixel kernels only
   long long last_tracked = 0;
   long long refined = 0;  // pixels re-evaluated in FP64 by the last split-BF16 call
   double last_kernel_ms = 0.;
@@ -104,6 +107,10 @@ extern "C" int flct_hip_create(flct_hip_ctx** out, int device) {
       return FLCT_HIP_E_HIP;
     }
   }
+  // kernel choice of the split-BF16 form: "pixel" keeps every pixel on the
+  // per-pixel kernels (for comparisons); default is automatic
+  if (const char* k = std::getenv("FLCT_HIP_BF16_KERNEL"))
+    c->bf16_kernel = (std::strcmp(k, "pixel") == 0) ? 1 : 0;
   *out = c;
   return FLCT_HIP_OK;
 }
@@ -114,7 +121,7 @@ extern "C" int flct_hip_destroy(flct_hip_ctx* c) {
   DevBuf* bufs[] = {&c->f1, &c->f2, &c->vx, &c->vy, &c->vm, &c->list, &c->count, &c->gw,
                     &c->filt, &c->absmax, &c->wtmp, &c->wF, &c->wtw1, &c->wtw2, &c->w3,
                     &c->w4, &c->w5, &c->hs1, &c->hs2, &c->mean1, &c->mean2, &c->flags,
-                    &c->fcount};
+                    &c->fcount, &c->segtab, &c->elist, &c->ecount};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (auto& e : c->ev)
@@ -193,6 +200,39 @@ extern "C" int flct_hip_check_opts(int nx, int ny, const flct_hip_opts* o, int* 
   if (o->precision < FLCT_HIP_FP64 || o->precision > FLCT_HIP_SPLIT_BF16) return FLCT_HIP_E_ARG;
   if (o->deltat == 0.) return FLCT_HIP_E_ARG;
   return FLCT_HIP_OK;
+}
+
+// Split-BF16 form through the row-segment kernel: interior columns by
+// segments of one row, clamped edge columns by the per-pixel kernel.
+static int track_bf16_seg(flct_hip_ctx* c, const LaunchCtx& lc, const TrackArgs& ta, double sigma,
+                          double thresh, int skip, int xoffset, int yoffset) {
+  const int N = 2 * ta.nsize;
+  int rc;
+  hipStream_t s = c->stream;
+  if (c->seg_n != N || c->seg_sigma != sigma) {
+    std::vector<unsigned short> tab;
+    seg_tables(N, sigma, tab);
+    if ((rc = ensure(c->segtab, tab.size() * 2))) return rc;
+    HIPCHK(hipMemcpyAsync(c->segtab.p, tab.data(), tab.size() * 2, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->seg_n = N;
+    c->seg_sigma = sigma;
+  }
+  const long long nedge_max = (ta.list ? ta.nwork : (long long)ta.nx * (N - 1));
+  if ((rc = ensure(c->elist, (size_t)(nedge_max > 0 ? nedge_max : 1) * sizeof(int))) ||
+      (rc = ensure(c->ecount, sizeof(unsigned int))))
+    return rc;
+  long long nedge = 0;
+  if ((rc = launch_edge_list(lc, ta.list, ta.nwork, ta.nx, ta.ny, ta.nsize, (int*)c->elist.p,
+                             (unsigned int*)c->ecount.p, &nedge)))
+    return rc;
+  if (nedge > 0) {
+    TrackArgs te = ta;
+    te.list = (const int*)c->elist.p;
+    te.nwork = nedge;
+    if ((rc = dispatch_track_bf16(N, lc, te))) return rc;
+  }
+  return launch_track_seg(lc, ta, c->segtab.p, thresh, skip, xoffset, yoffset);
 }
 
 static double ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -368,7 +408,10 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
   if (nwork > 0) {
     LaunchCtx lc{c->stream, c->num_cu, c->lds_per_block};
     if (o->precision == FLCT_HIP_SPLIT_BF16) {
-      rc = dispatch_track_bf16(N, lc, ta);
+      if (!c->bf16_kernel && seg_supported(N, nx, ny, o->bias))
+        rc = track_bf16_seg(c, lc, ta, o->sigma, thresh, r.skipon ? o->skip : 0, xoffset, yoffset);
+      else
+        rc = dispatch_track_bf16(N, lc, ta);
       if (!rc) {
         HIPCHK(hipEventRecord(c->ev[4], s));
         unsigned int nflag = 0;

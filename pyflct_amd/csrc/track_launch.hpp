@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/flct_hip.h"
 #include "track_kernels.hpp"
 
@@ -43,5 +45,17 @@ int dispatch_track_f32_lo(int M, const LaunchCtx& c, const TrackArgs& a);
 int dispatch_track_f32_hi(int M, const LaunchCtx& c, const TrackArgs& a);
 // split-BF16 MFMA form, box size N = 4..96
 int dispatch_track_bf16(int N, const LaunchCtx& c, const TrackArgs& a);
+
+// split-BF16 row-segment form: boxes N = 36..64 with N % 4 == 0, interior
+// columns, no bias pass.  seg_tables fills the split DFT tables for box N and
+// window width sigma (bf16 bits, copied to the device by the caller); launch_edge_list lists the tracked pixels of the clamped edge
+// columns (from a tracked-pixel list, or every edge pixel when list is null)
+// and returns their number through *nedge.
+bool seg_supported(int N, int nx, int ny, int bias);
+void seg_tables(int N, double sigma, std::vector<unsigned short>& tab);
+int launch_edge_list(const LaunchCtx& c, const int* list, long long nlist, int nx, int ny,
+                     int nsize, int* out, unsigned int* count, long long* nedge);
+int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, double thresh,
+                     int skip, int xoffset, int yoffset);
 
 }  // namespace flct

@@ -1,0 +1,713 @@
+// Split-BF16 MFMA tracker, row-segment form, for boxes N = 36..64 with
+// N % 4 == 0 and columns whose box is not clamped at the left or right edge.
+//
+// The windowed sub-image of pixel (i, j) is g(x, c) = Wr(x) Wc(c) (f(imin + x,
+// jmin + c) - fbar) (flct.c:815-870).  Every pixel of one image row shares
+// imin and the row window Wr, and for interior columns jmin = j - (nsize - 1)
+// and the column window Wc are the same up to a shift of the box.  So the
+// transform along the box rows is done once per segment of P <= 64
+// neighbouring pixels of a row, for every image column the segment's boxes
+// touch:
+//
+//   stage 1 (shared, x -> k):  T(u, k) = sum_x Wr(x) (f(imin + x, cb + u) - cm(u)) D(x, k),
+//            cm(u) the column's Wr-weighted mean over the box rows (so the
+//            windowed DC of every column is zero), D(x, k) = exp(-2 pi i k x / N)
+//            for k = 1 .. N/2-1; the real DC and Nyquist rows are packed into
+//            k = 0 as T_dc + i T_nyq by the table (D(x, 0) = 1 + i (-1)^x).  A
+//            column of ones (u = 127) gives sc Wr^(k).
+//   stage 2 (per pixel, c -> l, radix-2 on c, window folded into the tables):
+//            E(l) = sum_c' Wc(2c') T(p + 2c', k) e^{-2 pi i l c' / Nh},
+//            O(l) = sum_c' Wc(2c'+1) T(p + 2c' + 1, k) e^{-2 pi i l (2c'+1) / N},
+//            S(l) = E + O, S(l + Nh) = E - O, plus the mean correction
+//            Wr^(k) V(l), V the same transform of the column vector
+//            cm(p + c) - fbar (exact in real arithmetic; every term is of the
+//            size of the box's own variation, and a constant box gives S = 0
+//            exactly, the reference's max == 0 case).
+//   cross power P = conj(S1) S2 (flct.c:1461-1477; normfac dropped: the
+//            argmax and the fit are scale invariant), times filter^2
+//            (flct.c:1445-1457); the packed column is unpacked through l <-> -l.
+//   inverse 1 (l -> jj, radix-2 DIF): U(k, 2m'+q) from Q0 = P(l) + P(l+Nh) and
+//            Q1 = P(l) - P(l+Nh), the twiddle folded into the q = 1 table.
+//   inverse 2 (k -> ii, Hermitian half sum, radix-2 on ii):
+//            c(ii, jj) = Ce + Co, c(ii + Nh, jj) = Ce - Co over even / odd k.
+//
+// One wave tracks one pixel at a time.  Stage-2 outputs are rows l for the
+// lane's column k, which is exactly the A operand (row k, 8 contraction
+// values l) of inverse 1; inverse 1's outputs are rows k for the lane's column
+// jj, the A operand of inverse 2.  So after stage 1 nothing of a pixel passes
+// through LDS except the packed-column exchange and the 3x3 neighbourhood.
+// Products are hi*hi + hi*lo + lo*hi of BF16 splits accumulated in FP32
+// (v_mfma_f32_16x16x32_bf16); the argmax uses the values only, the 3x3 Taylor
+// fit (flct.c:1536-1556) is FP64.  Pixels the BF16 surface cannot settle (a
+// second cell within kTieTol of the maximum, an ill-conditioned fit) are
+// listed for FP64 re-evaluation as in the per-pixel kernels.
+#pragma once
+
+#include "track_mfma2.hpp"
+
+namespace flct {
+
+struct SegArgs {
+  TrackArgs t;           // images, box size, means, outputs, filter, flags, scales
+  const __bf16* tab;     // split tables, SegCfg layout (global)
+  double thresh;         // absolute threshold (flct.c:721-722), 0: off
+  int skip, xoffset, yoffset;  // flct.c:734-757 (transp = 1)
+  int jlo;               // first interior column, nsize - 1
+  int width;             // interior columns, ny - N + 1
+  int nband;             // column bands of at most 64 pixels
+  int nseg;              // nband * nx segments
+};
+
+struct SegCfg {
+  static constexpr int P = 64;            // pixels per segment, at most
+  static constexpr int kWaves = 8;
+  static constexpr int kThreads = 64 * kWaves;
+  static constexpr int RS = 40;           // table row stride (bf16), 32 + 8
+  static constexpr int kPlane = 32 * RS;
+  static constexpr int RS1 = 72;          // stage-1 table row stride, 64 + 8
+  static constexpr int kD1Plane = 32 * RS1;
+  // global table: D1 (4 planes, stays in global), then the LDS part
+  static constexpr int kLdsTabOff = 4 * kD1Plane;
+  // LDS tables (bf16 planes of 32 rows): De, Do (re_h, re_l, im_h, im_l,
+  // -im_h, -im_l), BI0, BI1 (re_h, re_l, im_h, im_l), WT (set 0 h, l, set 1 h, l)
+  static constexpr int kDe = 0, kDo = 6 * kPlane, kBI0 = 12 * kPlane, kBI1 = 16 * kPlane,
+                       kWT = 20 * kPlane;
+  static constexpr int kTabEl = 24 * kPlane;
+  static constexpr int kGlobalEl = kLdsTabOff + kTabEl;
+  // T planes [X][par][comp][k][uh], comp = re_h, re_l, im_h, im_l
+  static constexpr int RW = 66;           // >= 32 + 32 + 1 (odd reads), even
+  static constexpr int kTPlane = 32 * RW;
+  static constexpr int kTEl = 16 * kTPlane;
+  static constexpr int kScratch = 128 * 8 + 64 * 8 + 9 * 8 + 8;  // zx, zp, nb, pad
+  static constexpr int kLdsBytes = kTabEl * 2 + 256 * 4 + kTEl * 2 + 68 * 4 + kWaves * kScratch;
+};
+
+__device__ __forceinline__ bf16x8 u4_to_bf8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// 8 consecutive bf16 of an LDS row starting at element e (any alignment);
+// odd is e & 1, uniform over the wave
+__device__ __forceinline__ bf16x8 ld8_any(const __bf16* row, int e, bool odd) {
+  u32x4 r;
+  if (!odd) {
+    const unsigned* q = reinterpret_cast<const unsigned*>(row + e);
+    r[0] = q[0];
+    r[1] = q[1];
+    r[2] = q[2];
+    r[3] = q[3];
+  } else {
+    const unsigned* q = reinterpret_cast<const unsigned*>(row + e - 1);
+    const unsigned d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
+    r[0] = __builtin_amdgcn_alignbyte(d1, d0, 2);
+    r[1] = __builtin_amdgcn_alignbyte(d2, d1, 2);
+    r[2] = __builtin_amdgcn_alignbyte(d3, d2, 2);
+    r[3] = __builtin_amdgcn_alignbyte(d4, d3, 2);
+  }
+  return u4_to_bf8(r);
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+
+typedef const __attribute__((address_space(4))) SegArgs* SegArgsPtr;
+
+// The kernel's argument block, read through an opaque pointer: loads of the
+// fields are issued where they are used instead of being kept in scalar
+// registers across the segment and pixel loops.
+__device__ __forceinline__ SegArgsPtr seg_args() {
+  SegArgsPtr p = __builtin_bit_cast(SegArgsPtr, __builtin_amdgcn_kernarg_segment_ptr());
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+__global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
+  using C = SegCfg;
+  constexpr int RS = C::RS, PL = C::kPlane, RW = C::RW, TP = C::kTPlane;
+  extern __shared__ __align__(16) unsigned char lds[];
+  (void)A0;
+  SegArgsPtr Ak = seg_args();
+  const auto& A = *Ak;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int n0 = lane & 15, g0 = lane >> 4;
+
+  __bf16* TB = reinterpret_cast<__bf16*>(lds);
+  float* cmr = reinterpret_cast<float*>(TB + C::kTabEl);  // [X][u] cm(u) - mref
+  __bf16* TT = reinterpret_cast<__bf16*>(cmr + 256);
+  int* slist = reinterpret_cast<int*>(TT + C::kTEl);
+  int* scount = slist + 64;
+  unsigned char* wsc = reinterpret_cast<unsigned char*>(slist + 68) + wave * C::kScratch;
+  float2* zx = reinterpret_cast<float2*>(wsc);  // [2][64] packed-column spectra
+  float2* zp = zx + 128;                        // [64] packed cross power
+  double* nb = reinterpret_cast<double*>(zp + 64);
+
+  // ---------------- tables to LDS, T planes zeroed (padding stays zero)
+  {
+    const u32x4* src = reinterpret_cast<const u32x4*>(A.tab + C::kLdsTabOff);
+    u32x4* dst = reinterpret_cast<u32x4*>(TB);
+    for (int e = tid; e < C::kTabEl / 8; e += blockDim.x) dst[e] = src[e];
+    u32x4* tz = reinterpret_cast<u32x4*>(TT);
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    for (int e = tid; e < C::kTEl / 8; e += blockDim.x) tz[e] = z;
+  }
+
+  const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+  const int G = gridDim.x;
+  const int s0 = (int)((long long)blockIdx.x * A.nseg / G);
+  const int s1 = (int)((long long)(blockIdx.x + 1) * A.nseg / G);
+
+  for (int sidx = s0; sidx < s1; ++sidx) {
+    SegArgsPtr As = seg_args();
+    const auto& A = *As;
+    const auto& a = A.t;
+    const int nx = a.nx, ny = a.ny, nsize = a.nsize;
+    const int N = 2 * nsize, Nh = nsize;
+    const int ns = ny - N + 1;
+    // image scales: powers of two bringing max |f| to [1, 2)
+    double sc[2] = {1., 1.};
+    if (a.absmax) {
+      static_for<2>([&](auto X) {
+        const double m = __longlong_as_double((long long)a.absmax[X]);
+        if (m > 0. && m < 1e300) sc[X] = ldexp(1., -ilogb(m));
+      });
+    }
+    const int band = sidx / nx;
+    const int i = sidx - band * nx;
+    const int jb0 = A.jlo + (int)((long long)band * A.width / A.nband);
+    const int jb1 = A.jlo + (int)((long long)(band + 1) * A.width / A.nband);
+    const int P = jb1 - jb0;
+    // box rows (flct.c:773-779)
+    const int imin0 = (0 > (i - (nsize - 1))) ? 0 : i - (nsize - 1);
+    const int imax0 = ((nx - 1) < (i + nsize)) ? nx - 1 : i + nsize;
+    const int imin = (imax0 == nx - 1) ? nx - 1 - (2 * nsize - 1) : imin0;
+
+    __syncthreads();  // the previous segment's pixels are done with T and slist
+    // tracked pixels of the segment: threshold and skip rules (flct.c:721-757)
+    if (tid < 64) {
+      bool tr = false;
+      if (tid < P) {
+#pragma clang fp contract(off)
+        const int j = jb0 + tid;
+        const long long t = (long long)i * ny + j;
+        const double fabsbar = 0.5 * (fabs(a.f1[t] + a.f2[t]));
+        const bool below = (fabsbar < A.thresh);
+        bool noskip = true;
+        if (A.skip) noskip = (!((i - A.xoffset) % A.skip)) && (!((j - A.yoffset) % A.skip));
+        tr = (!below) && noskip;
+      }
+      const unsigned long long m = __ballot(tr);
+      if (tr) slist[__popcll(m & ((1ull << tid) - 1ull))] = tid;
+      if (tid == 0) *scount = __popcll(m);
+    }
+    __syncthreads();
+    const int cnt = __builtin_amdgcn_readfirstlane(*scount);
+    if (cnt == 0) continue;
+
+    const int cb = jb0 - (nsize - 1);  // image column of T column 0
+    const int ucols = P + N - 1;       // T columns the boxes touch
+    const double mref0 = a.mean1[(long long)imin * ns + cb];
+    const double mref1 = a.mean2[(long long)imin * ns + cb];
+
+    // ---------------- stage 1: wave w transforms T columns 16w .. 16w+15
+    {
+      // per-segment copies of the lane indices: keeps the table loads below
+      // from being hoisted out of the segment loop
+      const int n1 = fresh(n0), g1 = fresh(g0);
+      const int u = 16 * wave + n1;
+      const bool colok = u < ucols;
+      const bool ones = (u == 127);
+      const int ucl = colok ? u : 0;  // loads stay inside the image
+      // rows x = 32 s + 8 g + e of this lane
+      float wr[16];
+      static_for<16>([&](auto q) {
+        const int x = 32 * (q >> 3) + 8 * g1 + (q & 7);
+        wr[int(q)] = (x < N) ? (float)a.gw[(x < N ? x : 0) + imin - i + N - 1] : 0.f;
+      });
+      const __bf16* D1 = A.tab;
+      static_for<2>([&](auto X) {
+        const double* f = (X == 0) ? a.f1 : a.f2;
+        const double mref = (X == 0) ? mref0 : mref1;
+        const float scf = (float)sc[X];
+        bf16x8 Ah[2], Al[2];
+        double fv[16];
+        double csum = 0., wsum = 0.;
+        static_for<16>([&](auto q) {
+          const int x = 32 * (q >> 3) + 8 * g1 + (q & 7);
+          const int xr = (x < N) ? x : 0;
+          fv[int(q)] = f[(long long)(imin + xr) * ny + (cb + ucl)];
+          csum += (double)wr[int(q)] * fv[int(q)];  // wr is zero for x >= N
+          wsum += (double)wr[int(q)];
+        });
+        // window-weighted column mean over the box rows (the four lanes of a
+        // column hold 16 rows each): every column's windowed DC term is zero
+        csum += __shfl_xor(csum, 16);
+        csum += __shfl_xor(csum, 32);
+        wsum += __shfl_xor(wsum, 16);
+        wsum += __shfl_xor(wsum, 32);
+        const double cm = csum / wsum;
+        if (g1 == 0) cmr[128 * X + u] = colok ? (float)(cm - mref) : 0.f;
+        static_for<2>([&](auto s) {
+          float v[8];
+          static_for<8>([&](auto e) {
+            const float ws = wr[8 * s + e] * scf;  // zero for x >= N
+            const float gv = (float)(fv[8 * s + e] - cm) * ws;
+            v[int(e)] = ones ? ws : (colok ? gv : 0.f);
+          });
+          u32x4 hv, lv;
+          static_for<4>([&](auto d) {
+            unsigned hh, ll;
+            split2(v[2 * d], v[2 * d + 1], hh, ll);
+            hv[int(d)] = hh;
+            lv[int(d)] = ll;
+          });
+          Ah[s] = u4_to_bf8(hv);
+          Al[s] = u4_to_bf8(lv);
+        });
+        static_for<2>([&](auto kt) {
+          const int k = 16 * kt + n1;
+          f32x4 tre = z4, tim = z4;
+          static_for<2>([&](auto s) {
+            const int off = k * C::RS1 + 32 * s + 8 * g1;
+            const bf16x8 brh = ld8(D1 + 0 * C::kD1Plane + off), brl = ld8(D1 + 1 * C::kD1Plane + off);
+            const bf16x8 bih = ld8(D1 + 2 * C::kD1Plane + off), bil = ld8(D1 + 3 * C::kD1Plane + off);
+            tre = mfma3(Ah[s], Al[s], brh, brl, tre);
+            tim = mfma3(Ah[s], Al[s], bih, bil, tim);
+          });
+          // lane holds T(u = 16w + 4g + t, k = 16kt + n): parity t & 1,
+          // half column uh = 8w + 2g + (t >> 1)
+          const int uh = 8 * wave + 2 * g1;
+          static_for<2>([&](auto par) {
+            __bf16* pl = TT + ((X * 2 + par) * 4) * TP + k * RW + uh;
+            unsigned h0, l0, h1, l1;
+            split2(tre[int(par)], tre[int(par) + 2], h0, l0);
+            split2(tim[int(par)], tim[int(par) + 2], h1, l1);
+            *reinterpret_cast<unsigned*>(pl + 0 * TP) = h0;
+            *reinterpret_cast<unsigned*>(pl + 1 * TP) = l0;
+            *reinterpret_cast<unsigned*>(pl + 2 * TP) = h1;
+            *reinterpret_cast<unsigned*>(pl + 3 * TP) = l1;
+          });
+        });
+      });
+    }
+    __syncthreads();  // T complete
+
+    // sc_X Wr^(k) for the lane's k (T column 127: parity 1, uh 63)
+    float2 wrh[2][2];  // [X][kt]
+    static_for<2>([&](auto X) {
+      static_for<2>([&](auto kt) {
+        const __bf16* pl = TT + ((X * 2 + 1) * 4) * TP + (16 * kt + n0) * RW + 63;
+        const float re = (float)pl[0 * TP] + (float)pl[1 * TP];
+        const float im = (float)pl[2 * TP] + (float)pl[3 * TP];
+        wrh[X][kt] = make_float2(re, im);
+      });
+    });
+
+    // ---------------- pixels of the segment, one per wave at a time
+    for (int qi = wave; qi < cnt; qi += C::kWaves) {
+      SegArgsPtr Ap = seg_args();
+      const auto& a = Ap->t;
+      const int p = __builtin_amdgcn_readfirstlane(slist[qi]);
+      const int j = jb0 + p;
+      const int jmin = j - (nsize - 1);
+      // box-mean offsets from the segment's reference means (the ones
+      // column already carries the image scale)
+      const float dl[2] = {(float)(a.mean1[(long long)imin * ns + jmin] - mref0),
+                           (float)(a.mean2[(long long)imin * ns + jmin] - mref1)};
+      // per-pixel copies of lane indices: keeps index math out of registers
+      // across the pixel loop
+      const int n = fresh(n0), g = fresh(g0);
+      const int pe = p & 1, be = p >> 1;             // even columns c = 2c'
+      const int po = pe ^ 1, bo = (p + 1) >> 1;      // odd columns
+      const bool oe = (be & 1) != 0, oo = (bo & 1) != 0;
+
+      // A operands of stage 2 (window-folded half-DFT tables) for l-tile LT;
+      // the row index is refreshed per use so that the loads are not kept
+      // live across the loops below
+      struct DFr {
+        bf16x8 rh, rl, ih, il, nh, nl;
+      };
+      auto dtab = [&](int base, auto LT) {
+        const int row = fresh((16 * LT + n) * RS + 8 * g);
+        const __bf16* D = TB + base + row;
+        DFr d;
+        d.rh = ld8(D + 0 * PL);
+        d.rl = ld8(D + 1 * PL);
+        d.ih = ld8(D + 2 * PL);
+        d.il = ld8(D + 3 * PL);
+        d.nh = ld8(D + 4 * PL);
+        d.nl = ld8(D + 5 * PL);
+        return d;
+      };
+      // complex (D T) over the 32 half columns: re = Dr Tr - Di Ti, im = Dr Ti + Di Tr
+      auto cmul_mfma = [&](const DFr& d, bf16x8 trh, bf16x8 trl, bf16x8 tih, bf16x8 til,
+                           f32x4& re, f32x4& im) {
+        re = mfma3(d.rh, d.rl, trh, trl, z4);
+        re = mfma3(d.nh, d.nl, tih, til, re);
+        im = mfma3(d.rh, d.rl, tih, til, z4);
+        im = mfma3(d.ih, d.il, trh, trl, im);
+      };
+
+      CF4 Pp[2][2], Pm[2][2];  // [kt][LT] cross power
+      static_for<2>([&](auto LT) {
+        // mean correction V(l) of both images: the stage-2 transform of the
+        // real column vector cm(p + c) - fbar (same for every k)
+        CF4 Vp[2], Vm[2];
+        static_for<2>([&](auto X) {
+          f32x4 er, ei, orr, oi;
+          static_for<2>([&](auto sset) {
+            // v(c') = cm(p + 2c' + sset) - fbar_p, c' = 8g + e
+            const float* cr = cmr + 128 * X + p + sset + 16 * g;
+            float v[8];
+            static_for<8>([&](auto e) { v[int(e)] = cr[2 * e] - dl[X]; });
+            u32x4 hv, lv;
+            static_for<4>([&](auto d) {
+              unsigned hh, ll;
+              split2(v[2 * d], v[2 * d + 1], hh, ll);
+              hv[int(d)] = hh;
+              lv[int(d)] = ll;
+            });
+            const bf16x8 vh = u4_to_bf8(hv), vl = u4_to_bf8(lv);
+            const DFr d = dtab(sset == 0 ? C::kDe : C::kDo, LT);
+            f32x4& rr = (sset == 0) ? er : orr;
+            f32x4& ri = (sset == 0) ? ei : oi;
+            rr = mfma3(d.rh, d.rl, vh, vl, z4);
+            ri = mfma3(d.ih, d.il, vh, vl, z4);
+          });
+          Vp[X].re = er + orr;
+          Vp[X].im = ei + oi;
+          Vm[X].re = er - orr;
+          Vm[X].im = ei - oi;
+        });
+        static_for<2>([&](auto kt) {
+          const int k = 16 * kt + n;
+          // stage-2 tile of image X: S(l), S(l + Nh) for the lane's rows
+          auto stage2 = [&](auto X, CF4& sp, CF4& sm) {
+            f32x4 er, ei, orr, oi;
+            {
+              const __bf16* tr = TT + ((X * 2 + pe) * 4) * TP + k * RW + be + 8 * g;
+              const bf16x8 th = ld8_any(tr + 0 * TP, 0, oe), tl = ld8_any(tr + 1 * TP, 0, oe);
+              const bf16x8 uh = ld8_any(tr + 2 * TP, 0, oe), ul = ld8_any(tr + 3 * TP, 0, oe);
+              cmul_mfma(dtab(C::kDe, LT), th, tl, uh, ul, er, ei);
+            }
+            {
+              const __bf16* tr = TT + ((X * 2 + po) * 4) * TP + k * RW + bo + 8 * g;
+              const bf16x8 th = ld8_any(tr + 0 * TP, 0, oo), tl = ld8_any(tr + 1 * TP, 0, oo);
+              const bf16x8 uh = ld8_any(tr + 2 * TP, 0, oo), ul = ld8_any(tr + 3 * TP, 0, oo);
+              cmul_mfma(dtab(C::kDo, LT), th, tl, uh, ul, orr, oi);
+            }
+            const float2 w = wrh[X][kt];
+            static_for<4>([&](auto t) {
+              const float vpr = Vp[X].re[int(t)], vpi = Vp[X].im[int(t)];
+              const float vmr = Vm[X].re[int(t)], vmi = Vm[X].im[int(t)];
+              sp.re[int(t)] = (er[int(t)] + orr[int(t)]) + (w.x * vpr - w.y * vpi);
+              sp.im[int(t)] = (ei[int(t)] + oi[int(t)]) + (w.x * vpi + w.y * vpr);
+              sm.re[int(t)] = (er[int(t)] - orr[int(t)]) + (w.x * vmr - w.y * vmi);
+              sm.im[int(t)] = (ei[int(t)] - oi[int(t)]) + (w.x * vmi + w.y * vmr);
+            });
+            if constexpr (kt == 0) {
+              // the packed DC / Nyquist column to the exchange buffer
+              if (n == 0) {
+                static_for<4>([&](auto t) {
+                  const int l = 16 * LT + 4 * g + t;
+                  if (l < Nh) {
+                    zx[64 * X + l] = make_float2(sp.re[int(t)], sp.im[int(t)]);
+                    zx[64 * X + l + Nh] = make_float2(sm.re[int(t)], sm.im[int(t)]);
+                  }
+                });
+              }
+            }
+          };
+          CF4 s1p, s1m, s2p, s2m;
+          stage2(std::integral_constant<int, 0>{}, s1p, s1m);
+          stage2(std::integral_constant<int, 1>{}, s2p, s2m);
+          // cross power conj(S1) S2, optional filter^2
+          static_for<4>([&](auto t) {
+            float fp2 = 1.f, fm2 = 1.f;
+            if (a.filt) {
+              const int l = 16 * LT + 4 * g + t;
+              if (k < Nh && l < Nh) {
+                const float fp = (float)a.filt[k * N + l], fm = (float)a.filt[k * N + l + Nh];
+                fp2 = fp * fp;
+                fm2 = fm * fm;
+              }
+            }
+            {
+              const float ar = s1p.re[int(t)], ai = s1p.im[int(t)];
+              const float br = s2p.re[int(t)], bi = s2p.im[int(t)];
+              Pp[kt][LT].re[int(t)] = (ar * br + ai * bi) * fp2;
+              Pp[kt][LT].im[int(t)] = (ar * bi - ai * br) * fp2;
+            }
+            {
+              const float ar = s1m.re[int(t)], ai = s1m.im[int(t)];
+              const float br = s2m.re[int(t)], bi = s2m.im[int(t)];
+              Pm[kt][LT].re[int(t)] = (ar * br + ai * bi) * fm2;
+              Pm[kt][LT].im[int(t)] = (ar * bi - ai * br) * fm2;
+            }
+          });
+        });
+      });
+
+      // packed columns of both images, unpacked through the partner -l:
+      // A0 = (z + conj z(-l)) / 2, A_nyq = (z - conj z(-l)) / (2i)
+      wave_lds_fence();
+      if (lane < N) {
+        const int lp = (lane == 0) ? 0 : N - lane;
+        const float2 z1 = zx[lane], q1 = zx[lp], z2 = zx[64 + lane], q2 = zx[64 + lp];
+        const float a0r1 = 0.5f * (z1.x + q1.x), a0i1 = 0.5f * (z1.y - q1.y);
+        const float anr1 = 0.5f * (z1.y + q1.y), ani1 = -0.5f * (z1.x - q1.x);
+        const float a0r2 = 0.5f * (z2.x + q2.x), a0i2 = 0.5f * (z2.y - q2.y);
+        const float anr2 = 0.5f * (z2.y + q2.y), ani2 = -0.5f * (z2.x - q2.x);
+        float n0f = 1.f, nnf = 1.f;
+        if (a.filt) {
+          const float f0 = (float)a.filt[lane], fn = (float)a.filt[Nh * N + lane];
+          n0f = f0 * f0;
+          nnf = fn * fn;
+        }
+        const float p0r = (a0r1 * a0r2 + a0i1 * a0i2) * n0f;
+        const float p0i = (a0r1 * a0i2 - a0i1 * a0r2) * n0f;
+        const float pnr = (anr1 * anr2 + ani1 * ani2) * nnf;
+        const float pni = (anr1 * ani2 - ani1 * anr2) * nnf;
+        zp[lane] = make_float2(p0r - pni, p0i + pnr);
+      }
+      wave_lds_fence();
+      if (n == 0) {
+        static_for<2>([&](auto LT) {
+          static_for<4>([&](auto t) {
+            const int l = 16 * LT + 4 * g + t;
+            if (l < Nh) {
+              const float2 pp = zp[l], pm = zp[l + Nh];
+              Pp[0][LT].re[int(t)] = pp.x;
+              Pp[0][LT].im[int(t)] = pp.y;
+              Pm[0][LT].re[int(t)] = pm.x;
+              Pm[0][LT].im[int(t)] = pm.y;
+            }
+          });
+        });
+      }
+      wave_lds_fence();
+
+      // ---------------- inverse 1: A = Q_q (row k, contraction over l)
+      unsigned Uh[2][2][2][4], Ul[2][2][2][4];  // [kt][NT][q][t']
+      static_for<2>([&](auto kt) {
+        static_for<2>([&](auto q) {
+          u32x4 rh, rl, ih, il;
+          static_for<2>([&](auto LT) {
+            static_for<2>([&](auto th) {
+              float vr[2], vi[2];
+              static_for<2>([&](auto b) {
+                constexpr int t = 2 * decltype(th)::value + decltype(b)::value;
+                const float pr = Pp[kt][LT].re[t], pi = Pp[kt][LT].im[t];
+                const float mr = Pm[kt][LT].re[t], mi = Pm[kt][LT].im[t];
+                vr[int(b)] = (q == 0) ? pr + mr : pr - mr;
+                vi[int(b)] = (q == 0) ? pi + mi : pi - mi;
+              });
+              unsigned h0, l0, h1, l1;
+              split2(vr[0], vr[1], h0, l0);
+              split2(vi[0], vi[1], h1, l1);
+              rh[2 * LT + th] = h0;
+              rl[2 * LT + th] = l0;
+              ih[2 * LT + th] = h1;
+              il[2 * LT + th] = l1;
+            });
+          });
+          const bf16x8 qrh = u4_to_bf8(rh), qrl = u4_to_bf8(rl);
+          const bf16x8 qih = u4_to_bf8(ih), qil = u4_to_bf8(il);
+          const bf16x8 nqih = neg8(qih), nqil = neg8(qil);
+          static_for<2>([&](auto NT) {
+            const int row = fresh((16 * NT + n) * RS + 8 * g);
+            const __bf16* B = TB + ((q == 0) ? C::kBI0 : C::kBI1) + row;
+            const bf16x8 brh = ld8(B + 0 * PL), brl = ld8(B + 1 * PL);
+            const bf16x8 bih = ld8(B + 2 * PL), bil = ld8(B + 3 * PL);
+            f32x4 ur = mfma3(qrh, qrl, brh, brl, z4);
+            ur = mfma3(nqih, nqil, bih, bil, ur);
+            f32x4 ui = mfma3(qrh, qrl, bih, bil, z4);
+            ui = mfma3(qih, qil, brh, brl, ui);
+            static_for<4>([&](auto t) {
+              unsigned hh, ll;
+              split2(ur[int(t)], ui[int(t)], hh, ll);
+              Uh[kt][NT][q][int(t)] = hh;
+              Ul[kt][NT][q][int(t)] = ll;
+            });
+          });
+        });
+      });
+
+      // ---------------- inverse 2: c over (NT, q) column tiles of jj
+      f32x4 cv[2][2][2][2];  // [NT][q][IT][pm]
+      static_for<2>([&](auto NT) {
+        static_for<2>([&](auto q) {
+          bf16x8 ah[2], al[2];
+          static_for<2>([&](auto s) {
+            u32x4 vh, vl;
+            vh[0] = Uh[0][NT][q][s];
+            vh[1] = Uh[0][NT][q][2 + s];
+            vh[2] = Uh[1][NT][q][s];
+            vh[3] = Uh[1][NT][q][2 + s];
+            vl[0] = Ul[0][NT][q][s];
+            vl[1] = Ul[0][NT][q][2 + s];
+            vl[2] = Ul[1][NT][q][s];
+            vl[3] = Ul[1][NT][q][2 + s];
+            ah[s] = u4_to_bf8(vh);
+            al[s] = u4_to_bf8(vl);
+          });
+          static_for<2>([&](auto IT) {
+            const int row = (16 * IT + n) * RS + 8 * g;
+            const __bf16* W = TB + C::kWT + row;
+            const f32x4 ce = mfma3(ah[0], al[0], ld8(W + 0 * PL), ld8(W + 1 * PL), z4);
+            const f32x4 co = mfma3(ah[1], al[1], ld8(W + 2 * PL), ld8(W + 3 * PL), z4);
+            cv[NT][q][IT][0] = ce + co;
+            cv[NT][q][IT][1] = ce - co;
+          });
+        });
+      });
+
+      // ---------------- argmax on |c|, first strict maximum in the scan order
+      // cell (NT, q, IT, t, pm) of lane (n, g): ii = 16 IT + n, x = ii + pm Nh,
+      // jj = 32 NT + 8 g + 2 t + q; in fftshift order (flct.c:1490-1510) it
+      // sits at row sx = (x + Nh) mod N, column sy = (jj + Nh) mod N.
+      float lmax = 0.f;
+      static_for<2>([&](auto NT) {
+        static_for<2>([&](auto q) {
+          static_for<2>([&](auto IT) {
+            static_for<2>([&](auto pm) {
+              const f32x4 v = cv[NT][q][IT][pm];
+              lmax = fmaxf(lmax, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+            });
+          });
+        });
+      });
+      const float gmaxf = wave_max(lmax);
+      const float thr = gmaxf * (float)(1. - kTieTol);
+      // cells within kTieTol of the maximum: exactly one (the maximum) settles
+      // the peak; with more the pixel is re-evaluated in FP64 and the location
+      // below (the first such cell of each slot, smallest scan index) only
+      // has to be deterministic.  A zero surface is the max == 0 case.
+      int nge = 0;
+      unsigned loc = 0xffffffffu;
+      if (gmaxf > 0.f) {
+        static_for<2>([&](auto NT) {
+          static_for<2>([&](auto q) {
+            static_for<2>([&](auto IT) {
+              static_for<2>([&](auto pm) {
+                static_for<4>([&](auto t) {
+                  const unsigned long long m = __ballot(fabsf(cv[NT][q][IT][pm][int(t)]) >= thr);
+                  if (m) {
+                    nge += __popcll(m);
+                    const int ln = __builtin_ctzll(m);
+                    const int ii = 16 * IT + (ln & 15);
+                    const int sx = (pm == 0) ? ii + Nh : ii;
+                    const int jj = 32 * NT + 8 * (ln >> 4) + 2 * t + q;
+                    const int sy = (jj < Nh) ? jj + Nh : jj - Nh;
+                    const unsigned L = (unsigned)(sx * N + sy);
+                    loc = (L < loc) ? L : loc;
+                  }
+                });
+              });
+            });
+          });
+        });
+      }
+      const double gmax = (double)gmaxf;
+      const bool nonzero = (gmaxf != 0.f);
+      int ixmax = Nh, iymax = Nh;
+      if (nonzero) {
+        ixmax = (int)(loc / (unsigned)N);
+        iymax = (int)(loc - (unsigned)ixmax * (unsigned)N);
+      }
+      const bool fit = nonzero && ixmax > 0 && ixmax < N - 1 && iymax > 0 && iymax < N - 1;
+      if (fit) {
+        // gather the 3x3 neighbourhood into LDS
+        int dxv[2][2];
+        bool xin[2][2];
+        static_for<2>([&](auto IT) {
+          static_for<2>([&](auto pm) {
+            const int ii = 16 * IT + n;
+            const int sx = (pm == 0) ? ii + Nh : ii;
+            dxv[IT][pm] = sx - ixmax;
+            xin[IT][pm] = (ii < Nh) && (sx - ixmax >= -1) && (sx - ixmax <= 1);
+          });
+        });
+        static_for<2>([&](auto NT) {
+          static_for<2>([&](auto q) {
+            static_for<4>([&](auto t) {
+              const int jj = 32 * NT + 8 * g + 2 * t + q;
+              const int sy = (jj < Nh) ? jj + Nh : jj - Nh;
+              const int dy = sy - iymax;
+              if ((jj < N) && (dy >= -1) && (dy <= 1)) {
+                static_for<2>([&](auto IT) {
+                  static_for<2>([&](auto pm) {
+                    if (xin[IT][pm])
+                      nb[(dxv[IT][pm] + 1) * 3 + (dy + 1)] = (double)fabsf(cv[NT][q][IT][pm][int(t)]);
+                  });
+                });
+              }
+            });
+          });
+        });
+      }
+      wave_lds_fence();
+      bool flagged = nonzero && nge >= 2;
+      double shiftsubx = 0., shiftsuby = 0.;
+      if (fit) {
+#pragma clang fp contract(off)
+        const double c00 = nb[4];
+        const double fx = 0.5 * (nb[7] - nb[1]);
+        const double fy = 0.5 * (nb[5] - nb[3]);
+        const double fxx = (nb[7] + nb[1] - 2. * (c00));
+        const double fyy = (nb[5] + nb[3] - 2. * (c00));
+        const double fxy = 0.25 * (nb[8] + nb[0] - nb[6] - nb[2]);
+        shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
+        shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
+        const double det = fxx * fyy - fxy * fxy;
+        const double hn = fabs(fxx) + fabs(fyy) + 2. * fabs(fxy);
+        const double bound = kRelErr * gmax * hn * (1. + 4. * (fabs(shiftsubx) + fabs(shiftsuby)));
+        if (!(bound <= kFitTol * fabs(det))) flagged = true;
+      }
+      wave_lds_fence();  // nb is rewritten by the next pixel
+      if (lane == 0) {
+        const double shiftx = ((double)ixmax + shiftsubx) - (double)Nh;
+        const double shifty = ((double)iymax + shiftsuby) - (double)Nh;
+        const long long pix = (long long)i * ny + j;
+        a.vx[pix] = shifty * a.deltinv * a.deltas;
+        a.vy[pix] = shiftx * a.deltinv * a.deltas;
+        a.vm[pix] = 1.;
+        if (flagged && a.flag_list) a.flag_list[atomicAdd(a.flag_count, 1u)] = (int)pix;
+      }
+    }
+  }
+}
+
+// Edge pixels for the per-pixel kernel: tracked pixels whose box is clamped
+// at the left or right image edge (columns outside [jlo, jhi]).  With a
+// tracked-pixel list (threshold or skip) the list is filtered; otherwise the
+// edge columns of every row are enumerated.
+__global__ void k_edge_list(const int* __restrict__ list, long long nlist, int nx, int ny, int jlo,
+                            int jhi, int* __restrict__ out, unsigned int* __restrict__ count) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (list) {
+    bool e = false;
+    int p = 0;
+    if (t < nlist) {
+      p = list[t];
+      const int j = p % ny;
+      e = (j < jlo) || (j > jhi);
+    }
+    const unsigned long long mask = __ballot(e);
+    const int ln = threadIdx.x & 63;
+    unsigned int base = 0;
+    if (ln == 0 && mask) base = atomicAdd(count, (unsigned)__popcll(mask));
+    base = __shfl(base, 0);
+    if (e) out[base + __popcll(mask & ((1ull << ln) - 1ull))] = p;
+  } else {
+    const int we = jlo + (ny - 1 - jhi);
+    if (t < (long long)nx * we) {
+      const int r = (int)(t / we), c = (int)(t - (long long)r * we);
+      const int j = (c < jlo) ? c : jhi + 1 + (c - jlo);
+      out[t] = r * ny + j;
+    }
+  }
+}
+
+}  // namespace flct

@@ -18,8 +18,9 @@ pytestmark = pytest.mark.gpu
 RMS_TOL = 1e-4   # px, RMS over all tracked pixels
 MAX_TOL = 5e-3   # px, any pixel
 
-# (n, sigma, options): box sizes 20 and 84 use the dense-DFT kernel, 36..64
-# the radix-2 kernel
+# (n, sigma, options): box sizes 20 and 84 use the dense-DFT kernel; 36..64
+# the row-segment kernel for interior columns and the radix-2 kernel for the
+# clamped edge columns and the bias pass
 CASES = [
     (192, 5.0, {}),                         # N = 20
     (256, 9.0, {}),                         # N = 36
@@ -82,3 +83,56 @@ def test_bf16_other_forms_report_no_refinement(ctx):
     f1, f2, _ = synth.cfg4(n=192)
     ctx.track(f1, f2, 1.0, 1.0, 9.0)
     assert ctx.last_stats()["refined"] == 0
+
+
+# non-square and odd shapes for the row-segment kernel: narrow interior
+# widths (ny - N + 1), partial last bands, edge rows and columns
+SEG_SHAPES = [
+    (200, 70, 15.0, {}),                          # N = 44, 27 interior columns
+    (97, 301, 15.0, {}),                          # N = 64, bands of 60 and 61 columns
+    (150, 131, 10.0, {"kr": 0.4}),                # N = 40 with the filter
+    (180, 210, 15.0, {"thresh": 0.3}),            # relative threshold
+    (170, 190, 11.0, {"skip": 4, "poff": 1, "qoff": -2, "interp": True}),  # N = 44 lattice
+]
+
+
+@pytest.mark.parametrize("nx,ny,sigma,kw", SEG_SHAPES,
+                         ids=[f"{c[0]}x{c[1]}-s{c[2]:g}" for c in SEG_SHAPES])
+def test_bf16_segment_shapes(ctx, nx, ny, sigma, kw):
+    rng = np.random.default_rng(nx * 1000 + ny)
+    from conftest import smooth_pair
+    a, b = smooth_pair(rng, nx, ny, shift=(0.7, -1.2), width=2.5)
+    ref = ctx.track(a, b, 1.0, 1.0, sigma, **kw)
+    got = ctx.track(a, b, 1.0, 1.0, sigma, precision="split_bf16", **kw)
+    assert np.array_equal(got[2], ref[2])
+    m = ref[2] == 1.0
+    rms, mx = _errs((ref[0][m], ref[1][m]), (got[0][m], got[1][m]))
+    assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
+    # thresholded and unfilled pixels carry the reference's values exactly;
+    # filled pixels (vm = 0.5) interpolate the tracked velocities
+    z = ref[2] == 0.0
+    assert np.array_equal(got[0][z], ref[0][z]) and np.array_equal(got[1][z], ref[1][z])
+    h = ref[2] == 0.5
+    if h.any():
+        rms, mx = _errs((ref[0][h], ref[1][h]), (got[0][h], got[1][h]))
+        assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
+
+
+def test_bf16_segment_matches_pixel_kernel(monkeypatch):
+    """The row-segment kernel and the per-pixel radix-2 kernel compute the
+    same surfaces in a different order: their velocities agree to the split
+    precision."""
+    from pyflct_amd import api
+    f1, f2, _ = synth.cfg4(n=384)
+    seg = api.Context(0)
+    monkeypatch.setenv("FLCT_HIP_BF16_KERNEL", "pixel")
+    pix = api.Context(0)
+    try:
+        g = seg.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
+        p = pix.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
+    finally:
+        seg.close()
+        pix.close()
+    assert np.array_equal(g[2], p[2])
+    rms, mx = _errs(g, p)
+    assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
