@@ -39,12 +39,22 @@ def w_gemm(n):
     return 3.0 * (2 * (4 * n ** 3 + 8 * n * n * h) + (8 * n * n * h + 4 * n ** 3))
 
 
+MFMA_FLOPS = 2 * 16 * 16 * 32  # one v_mfma_f32_16x16x32_bf16
+
+
+def seg_mfma_flops(n):
+    """MFMA flops the row-segment split-BF16 kernel executes per interior
+    pixel (boxes 36..64): 128 three-product groups per pixel (stage 2: 64,
+    mean correction: 16, inverse 1: 32, inverse 2: 16), plus the shared
+    box-row transform, 128 groups per 64-pixel segment."""
+    return (128 + 128 / 64) * 3 * MFMA_FLOPS
+
+
 def r2_mfma_flops(n, bias=False):
-    """MFMA flops the radix-2 split-BF16 kernel executes per tracked pixel
-    (boxes 36..64): per wave and pass 88 three-product groups of
-    16x16x32 BF16 MFMAs (stage 1: 32, stage 2: 32, inverse 1: 16, inverse 2:
-    8), two waves per pixel."""
-    return 88 * 3 * (2 * 16 * 16 * 32) * 2 * (2 if bias else 1)
+    """MFMA flops the per-pixel radix-2 kernel executes per tracked pixel
+    (clamped edge columns and the bias pass): per wave and pass 88
+    three-product groups, two waves per pixel."""
+    return 88 * 3 * MFMA_FLOPS * 2 * (2 if bias else 1)
 
 
 def log(*a):
@@ -165,7 +175,8 @@ def main():
         peak, per_px, kname = PEAK_FP32_TFLOPS, w_fft(nbox), f"k_track<{nbox // 4},float>"
     else:
         peak, per_px = PEAK_BF16_TFLOPS, w_gemm(nbox)
-        kname = "k_track_r2<2>" if 36 <= nbox <= 64 else "k_track_mfma"
+        kname = ("k_track_seg (interior columns) + k_track_r2<2> (edge columns)"
+                 if 36 <= nbox <= 64 else "k_track_mfma")
     flops = per_px * hr["tracked"]
     achieved = flops / (hr["main_ms"] * 1e-3) / 1e12
     traffic = None
@@ -178,9 +189,10 @@ def main():
             "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": kname,
             "flops_per_px": per_px, "kernel_ms": round(hr["main_ms"], 3)}
     if head == "split_bf16" and 36 <= nbox <= 64:
-        ex = r2_mfma_flops(nbox) * hr["tracked"] / (hr["main_ms"] * 1e-3) / 1e12
-        roof["executed_mfma_flops_per_px"] = r2_mfma_flops(nbox)
-        roof["mfma_util"] = round(ex / peak, 4)
+        nedge = nx * (nbox - 1)  # clamped edge columns, per-pixel kernel
+        ex_flops = seg_mfma_flops(nbox) * (hr["tracked"] - nedge) + r2_mfma_flops(nbox) * nedge
+        roof["executed_mfma_flops_per_px"] = round(ex_flops / hr["tracked"])
+        roof["mfma_util"] = round(ex_flops / (hr["main_ms"] * 1e-3) / 1e12 / peak, 4)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -60,7 +60,11 @@ struct SegArgs {
 
 struct SegCfg {
   static constexpr int P = 64;            // pixels per segment, at most
+#ifdef FLCT_SEG_WAVES
+  static constexpr int kWaves = FLCT_SEG_WAVES;  // waves per block (tuning builds)
+#else
   static constexpr int kWaves = 8;
+#endif
   static constexpr int kThreads = 64 * kWaves;
   static constexpr int RS = 40;           // table row stride (bf16), 32 + 8
   static constexpr int kPlane = 32 * RS;
@@ -74,10 +78,14 @@ struct SegCfg {
                        kWT = 20 * kPlane;
   static constexpr int kTabEl = 24 * kPlane;
   static constexpr int kGlobalEl = kLdsTabOff + kTabEl;
-  // T planes [X][par][comp][k][uh], comp = re_h, re_l, im_h, im_l
-  static constexpr int RW = 66;           // >= 32 + 32 + 1 (odd reads), even
-  static constexpr int kTPlane = 32 * RW;
-  static constexpr int kTEl = 16 * kTPlane;
+  // T planes [X][par][k][comp][uh], comp = re_h, re_l, im_h, im_l.  A k row
+  // is 4 runs of RW half columns plus padding: 138 dwords, so the 16 rows a
+  // fragment read touches start 10 banks apart and a lane's dword pair never
+  // shares a bank with another lane's
+  static constexpr int RW = 68;           // >= 32 + 32 + 1 (odd reads), even
+  static constexpr int RK = 4 * RW + 4;   // k row stride
+  static constexpr int kTPlane = 32 * RK;
+  static constexpr int kTEl = 4 * kTPlane;
   static constexpr int kScratch = 128 * 8 + 64 * 8 + 9 * 8 + 8;  // zx, zp, nb, pad
   static constexpr int kLdsBytes = kTabEl * 2 + 256 * 4 + kTEl * 2 + 68 * 4 + kWaves * kScratch;
 };
@@ -123,7 +131,7 @@ __device__ __forceinline__ SegArgsPtr seg_args() {
 
 __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
   using C = SegCfg;
-  constexpr int RS = C::RS, PL = C::kPlane, RW = C::RW, TP = C::kTPlane;
+  constexpr int RS = C::RS, PL = C::kPlane, RW = C::RW, RK = C::RK, TP = C::kTPlane;
   extern __shared__ __align__(16) unsigned char lds[];
   (void)A0;
   SegArgsPtr Ak = seg_args();
@@ -210,12 +218,12 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
     const double mref0 = a.mean1[(long long)imin * ns + cb];
     const double mref1 = a.mean2[(long long)imin * ns + cb];
 
-    // ---------------- stage 1: wave w transforms T columns 16w .. 16w+15
-    {
+    // ---------------- stage 1: column tile ct (T columns 16ct .. 16ct+15)
+    for (int ct = wave; ct < 8; ct += C::kWaves) {
       // per-segment copies of the lane indices: keeps the table loads below
       // from being hoisted out of the segment loop
       const int n1 = fresh(n0), g1 = fresh(g0);
-      const int u = 16 * wave + n1;
+      const int u = 16 * ct + n1;
       const bool colok = u < ucols;
       const bool ones = (u == 127);
       const int ucl = colok ? u : 0;  // loads stay inside the image
@@ -277,16 +285,16 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           });
           // lane holds T(u = 16w + 4g + t, k = 16kt + n): parity t & 1,
           // half column uh = 8w + 2g + (t >> 1)
-          const int uh = 8 * wave + 2 * g1;
+          const int uh = 8 * ct + 2 * g1;
           static_for<2>([&](auto par) {
-            __bf16* pl = TT + ((X * 2 + par) * 4) * TP + k * RW + uh;
+            __bf16* pl = TT + (X * 2 + par) * TP + k * RK + uh;
             unsigned h0, l0, h1, l1;
             split2(tre[int(par)], tre[int(par) + 2], h0, l0);
             split2(tim[int(par)], tim[int(par) + 2], h1, l1);
-            *reinterpret_cast<unsigned*>(pl + 0 * TP) = h0;
-            *reinterpret_cast<unsigned*>(pl + 1 * TP) = l0;
-            *reinterpret_cast<unsigned*>(pl + 2 * TP) = h1;
-            *reinterpret_cast<unsigned*>(pl + 3 * TP) = l1;
+            *reinterpret_cast<unsigned*>(pl + 0 * RW) = h0;
+            *reinterpret_cast<unsigned*>(pl + 1 * RW) = l0;
+            *reinterpret_cast<unsigned*>(pl + 2 * RW) = h1;
+            *reinterpret_cast<unsigned*>(pl + 3 * RW) = l1;
           });
         });
       });
@@ -297,9 +305,9 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
     float2 wrh[2][2];  // [X][kt]
     static_for<2>([&](auto X) {
       static_for<2>([&](auto kt) {
-        const __bf16* pl = TT + ((X * 2 + 1) * 4) * TP + (16 * kt + n0) * RW + 63;
-        const float re = (float)pl[0 * TP] + (float)pl[1 * TP];
-        const float im = (float)pl[2 * TP] + (float)pl[3 * TP];
+        const __bf16* pl = TT + (X * 2 + 1) * TP + (16 * kt + n0) * RK + 63;
+        const float re = (float)pl[0 * RW] + (float)pl[1 * RW];
+        const float im = (float)pl[2 * RW] + (float)pl[3 * RW];
         wrh[X][kt] = make_float2(re, im);
       });
     });
@@ -318,6 +326,10 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       // per-pixel copies of lane indices: keeps index math out of registers
       // across the pixel loop
       const int n = fresh(n0), g = fresh(g0);
+      // lane bases: table row n, T row k = n, 8 contraction values from 8g;
+      // every operand below is an immediate offset from one of them
+      const __bf16* TBl = TB + (n * RS + 8 * g);
+      const __bf16* TTl = TT + (n * RK + 8 * g);
       const int pe = p & 1, be = p >> 1;             // even columns c = 2c'
       const int po = pe ^ 1, bo = (p + 1) >> 1;      // odd columns
       const bool oe = (be & 1) != 0, oo = (bo & 1) != 0;
@@ -329,8 +341,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         bf16x8 rh, rl, ih, il, nh, nl;
       };
       auto dtab = [&](int base, auto LT) {
-        const int row = fresh((16 * LT + n) * RS + 8 * g);
-        const __bf16* D = TB + base + row;
+        const __bf16* D = TBl + base + 16 * LT * RS;
         DFr d;
         d.rh = ld8(D + 0 * PL);
         d.rl = ld8(D + 1 * PL);
@@ -351,6 +362,8 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
 
       CF4 Pp[2][2], Pm[2][2];  // [kt][LT] cross power
       static_for<2>([&](auto LT) {
+        // table operands are re-read rather than held across the loops
+        asm volatile("" ::: "memory");
         // mean correction V(l) of both images: the stage-2 transform of the
         // real column vector cm(p + c) - fbar (same for every k)
         CF4 Vp[2], Vm[2];
@@ -381,20 +394,21 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           Vm[X].im = ei - oi;
         });
         static_for<2>([&](auto kt) {
+          asm volatile("" ::: "memory");
           const int k = 16 * kt + n;
           // stage-2 tile of image X: S(l), S(l + Nh) for the lane's rows
           auto stage2 = [&](auto X, CF4& sp, CF4& sm) {
             f32x4 er, ei, orr, oi;
             {
-              const __bf16* tr = TT + ((X * 2 + pe) * 4) * TP + k * RW + be + 8 * g;
-              const bf16x8 th = ld8_any(tr + 0 * TP, 0, oe), tl = ld8_any(tr + 1 * TP, 0, oe);
-              const bf16x8 uh = ld8_any(tr + 2 * TP, 0, oe), ul = ld8_any(tr + 3 * TP, 0, oe);
+              const __bf16* tr = TTl + (X * 2 + pe) * TP + 16 * kt * RK + be;
+              const bf16x8 th = ld8_any(tr + 0 * RW, 0, oe), tl = ld8_any(tr + 1 * RW, 0, oe);
+              const bf16x8 uh = ld8_any(tr + 2 * RW, 0, oe), ul = ld8_any(tr + 3 * RW, 0, oe);
               cmul_mfma(dtab(C::kDe, LT), th, tl, uh, ul, er, ei);
             }
             {
-              const __bf16* tr = TT + ((X * 2 + po) * 4) * TP + k * RW + bo + 8 * g;
-              const bf16x8 th = ld8_any(tr + 0 * TP, 0, oo), tl = ld8_any(tr + 1 * TP, 0, oo);
-              const bf16x8 uh = ld8_any(tr + 2 * TP, 0, oo), ul = ld8_any(tr + 3 * TP, 0, oo);
+              const __bf16* tr = TTl + (X * 2 + po) * TP + 16 * kt * RK + bo;
+              const bf16x8 th = ld8_any(tr + 0 * RW, 0, oo), tl = ld8_any(tr + 1 * RW, 0, oo);
+              const bf16x8 uh = ld8_any(tr + 2 * RW, 0, oo), ul = ld8_any(tr + 3 * RW, 0, oo);
               cmul_mfma(dtab(C::kDo, LT), th, tl, uh, ul, orr, oi);
             }
             const float2 w = wrh[X][kt];
@@ -422,30 +436,34 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           CF4 s1p, s1m, s2p, s2m;
           stage2(std::integral_constant<int, 0>{}, s1p, s1m);
           stage2(std::integral_constant<int, 1>{}, s2p, s2m);
-          // cross power conj(S1) S2, optional filter^2
+          // cross power conj(S1) S2
           static_for<4>([&](auto t) {
-            float fp2 = 1.f, fm2 = 1.f;
-            if (a.filt) {
-              const int l = 16 * LT + 4 * g + t;
-              if (k < Nh && l < Nh) {
-                const float fp = (float)a.filt[k * N + l], fm = (float)a.filt[k * N + l + Nh];
-                fp2 = fp * fp;
-                fm2 = fm * fm;
-              }
-            }
             {
               const float ar = s1p.re[int(t)], ai = s1p.im[int(t)];
               const float br = s2p.re[int(t)], bi = s2p.im[int(t)];
-              Pp[kt][LT].re[int(t)] = (ar * br + ai * bi) * fp2;
-              Pp[kt][LT].im[int(t)] = (ar * bi - ai * br) * fp2;
+              Pp[kt][LT].re[int(t)] = ar * br + ai * bi;
+              Pp[kt][LT].im[int(t)] = ar * bi - ai * br;
             }
             {
               const float ar = s1m.re[int(t)], ai = s1m.im[int(t)];
               const float br = s2m.re[int(t)], bi = s2m.im[int(t)];
-              Pm[kt][LT].re[int(t)] = (ar * br + ai * bi) * fm2;
-              Pm[kt][LT].im[int(t)] = (ar * bi - ai * br) * fm2;
+              Pm[kt][LT].re[int(t)] = ar * br + ai * bi;
+              Pm[kt][LT].im[int(t)] = ar * bi - ai * br;
             }
           });
+          if (a.filt) {
+            // filter^2 on the cross power (flct.c:1445-1457)
+            static_for<4>([&](auto t) {
+              const int l = 16 * LT + 4 * g + t;
+              if (k < Nh && l < Nh) {
+                const float fp = (float)a.filt[k * N + l], fm = (float)a.filt[k * N + l + Nh];
+                Pp[kt][LT].re[int(t)] *= fp * fp;
+                Pp[kt][LT].im[int(t)] *= fp * fp;
+                Pm[kt][LT].re[int(t)] *= fm * fm;
+                Pm[kt][LT].im[int(t)] *= fm * fm;
+              }
+            });
+          }
         });
       });
 
@@ -491,6 +509,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       // ---------------- inverse 1: A = Q_q (row k, contraction over l)
       unsigned Uh[2][2][2][4], Ul[2][2][2][4];  // [kt][NT][q][t']
       static_for<2>([&](auto kt) {
+        asm volatile("" ::: "memory");
         static_for<2>([&](auto q) {
           u32x4 rh, rl, ih, il;
           static_for<2>([&](auto LT) {
@@ -516,8 +535,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           const bf16x8 qih = u4_to_bf8(ih), qil = u4_to_bf8(il);
           const bf16x8 nqih = neg8(qih), nqil = neg8(qil);
           static_for<2>([&](auto NT) {
-            const int row = fresh((16 * NT + n) * RS + 8 * g);
-            const __bf16* B = TB + ((q == 0) ? C::kBI0 : C::kBI1) + row;
+            const __bf16* B = TBl + ((q == 0) ? C::kBI0 : C::kBI1) + 16 * NT * RS;
             const bf16x8 brh = ld8(B + 0 * PL), brl = ld8(B + 1 * PL);
             const bf16x8 bih = ld8(B + 2 * PL), bil = ld8(B + 3 * PL);
             f32x4 ur = mfma3(qrh, qrl, brh, brl, z4);
@@ -553,8 +571,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
             al[s] = u4_to_bf8(vl);
           });
           static_for<2>([&](auto IT) {
-            const int row = (16 * IT + n) * RS + 8 * g;
-            const __bf16* W = TB + C::kWT + row;
+            const __bf16* W = TBl + C::kWT + 16 * IT * RS;
             const f32x4 ce = mfma3(ah[0], al[0], ld8(W + 0 * PL), ld8(W + 1 * PL), z4);
             const f32x4 co = mfma3(ah[1], al[1], ld8(W + 2 * PL), ld8(W + 3 * PL), z4);
             cv[NT][q][IT][0] = ce + co;
