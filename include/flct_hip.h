@@ -94,6 +94,31 @@ int flct_hip_track_device(flct_hip_ctx *ctx, const double *d_f1, const double *d
                           int ny, const flct_hip_opts *opts, double *d_vx, double *d_vy,
                           double *d_vm);
 
+/* One image pair of a batch: device arrays (C-space, nx*ny doubles) and the
+ * options of its own flct run; tracked, refined and status are outputs. */
+typedef struct flct_hip_pair {
+  const double *f1, *f2;
+  double *vx, *vy, *vm;
+  int nx, ny;
+  flct_hip_opts opts;
+  long long tracked;  /* out: pixels whose cross-correlation was computed */
+  long long refined;  /* out: split-BF16 pixels re-evaluated in FP64 */
+  int status;         /* out: FLCT_HIP_OK or the pair's option error */
+} flct_hip_pair;
+
+/*
+ * flct_hip_track_batch runs flct_hip_track_device for every pair: the
+ * equivalent of npairs invocations of the reference flct (flct.c:261-1154,
+ * one pair per process), each with its own size and options.  All pairs'
+ * kernels are captured into one HIP graph on the context's stream and
+ * launched once; the graph is kept and relaunched while the next call
+ * brings the same pairs (arrays, sizes, options).  Every pair is validated
+ * first: if any is invalid, nothing runs, each pair's status is set and the
+ * first error is returned.  flct_hip_last_stats reports the sum of the
+ * tracked pixels and the device time of the graph.
+ */
+int flct_hip_track_batch(flct_hip_ctx *ctx, flct_hip_pair *pairs, int npairs);
+
 /* Number of pixels whose cross-correlation the last track call computed
  * (not thresholded, not skipped), and the device time of its tracking
  * kernel in milliseconds (HIP events on the context's stream). */

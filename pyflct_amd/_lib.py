@@ -40,6 +40,7 @@ EXPORTS = (
     "flct_hip_destroy",
     "flct_hip_track",
     "flct_hip_track_device",
+    "flct_hip_track_batch",
     "flct_hip_last_stats",
     "flct_hip_last_refined",
     "flct_hip_last_kernel_ms",
@@ -79,6 +80,23 @@ class Opts(ctypes.Structure):
     ]
 
 
+class Pair(ctypes.Structure):
+    """flct_hip_pair: one image pair of a batch (device arrays)."""
+    _fields_ = [
+        ("f1", ctypes.c_void_p),
+        ("f2", ctypes.c_void_p),
+        ("vx", ctypes.c_void_p),
+        ("vy", ctypes.c_void_p),
+        ("vm", ctypes.c_void_p),
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("opts", Opts),
+        ("tracked", ctypes.c_longlong),
+        ("refined", ctypes.c_longlong),
+        ("status", ctypes.c_int),
+    ]
+
+
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -100,6 +118,7 @@ def lib():
         L.flct_hip_destroy.argtypes = [vp]
         L.flct_hip_track.argtypes = [vp, _dp, _dp, i, i, ctypes.POINTER(Opts), _dp, _dp, _dp]
         L.flct_hip_track_device.argtypes = [vp, vp, vp, i, i, ctypes.POINTER(Opts), vp, vp, vp]
+        L.flct_hip_track_batch.argtypes = [vp, ctypes.POINTER(Pair), i]
         L.flct_hip_last_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong),
                                           ctypes.POINTER(d), ctypes.POINTER(d)]
         L.flct_hip_last_kernel_ms.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d)]

@@ -154,6 +154,60 @@ class Context:
         check(_lib.lib().flct_hip_track_device(self._h, d_f1, d_f2, nx, ny, ctypes.byref(opts),
                                                d_vx, d_vy, d_vm), "flct_hip_track_device")
 
+    def track_batch(self, pairs, **common):
+        """Track several image pairs in one graph launch (flct_hip_track_batch).
+
+        ``pairs`` is a sequence of ``(f1, f2, deltat, deltas, sigma)`` or
+        ``(f1, f2, deltat, deltas, sigma, options)`` tuples of host arrays;
+        ``options`` (a dict) overrides ``common``.  Returns a list of
+        ``(vx, vy, vm)``.  Buffers are allocated per call; use
+        :meth:`track_batch_device` to reuse device arrays (and the captured
+        graph) across calls.
+        """
+        bufs, descs = [], []
+        try:
+            for item in pairs:
+                f1, f2, dt, ds, sigma = item[:5]
+                kw = dict(common)
+                if len(item) > 5:
+                    kw.update(item[5])
+                f1, f2 = f64c(f1), f64c(f2)
+                if f1.ndim != 2 or f1.shape != f2.shape:
+                    raise ValueError("f1 and f2 must be 2-D arrays of the same shape")
+                nx, ny = f1.shape
+                d = [self.malloc(f1.nbytes) for _ in range(5)]
+                bufs.append(d)
+                self.upload(d[0], f1)
+                self.upload(d[1], f2)
+                descs.append((d, nx, ny, self.make_opts(dt, ds, sigma, **kw)))
+            self.track_batch_device([(d[0], d[1], d[2], d[3], d[4], nx, ny, o)
+                                     for d, nx, ny, o in descs])
+            out = []
+            for d, nx, ny, _ in descs:
+                out.append(tuple(self.download(np.empty((nx, ny)), d[k]) for k in (2, 3, 4)))
+            return out
+        finally:
+            for d in bufs:
+                for p in d:
+                    self.free(p)
+
+    def track_batch_device(self, pairs):
+        """``pairs``: sequence of (d_f1, d_f2, d_vx, d_vy, d_vm, nx, ny, opts)
+        with device pointers and :class:`Opts`.  Returns per-pair
+        ``{"tracked", "refined"}``; raises FlctHipError with the first failing
+        pair's status."""
+        arr = (_lib.Pair * len(pairs))()
+        for k, (f1, f2, vx, vy, vm, nx, ny, o) in enumerate(pairs):
+            arr[k].f1, arr[k].f2 = _vp(f1), _vp(f2)
+            arr[k].vx, arr[k].vy, arr[k].vm = _vp(vx), _vp(vy), _vp(vm)
+            arr[k].nx, arr[k].ny = int(nx), int(ny)
+            arr[k].opts = o
+        rc = _lib.lib().flct_hip_track_batch(self._h, arr, len(pairs))
+        if rc != _lib.FLCT_HIP_OK:
+            bad = [k for k in range(len(pairs)) if arr[k].status != _lib.FLCT_HIP_OK]
+            raise FlctHipError(rc, f"flct_hip_track_batch (pairs {bad})")
+        return [{"tracked": arr[k].tracked, "refined": arr[k].refined} for k in range(len(pairs))]
+
     def last_stats(self):
         n = ctypes.c_longlong()
         k = ctypes.c_double()
@@ -205,6 +259,10 @@ class Context:
         check(_lib.lib().flct_hip_download(self._h, a.ctypes.data_as(ctypes.c_void_p), dptr,
                                            a.nbytes), "flct_hip_download")
         return a
+
+
+def _vp(p):
+    return p.value if isinstance(p, ctypes.c_void_p) else p
 
 
 _default = None

@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "track_kernels.hpp"
+
 namespace flct {
 
 constexpr double kXmiss = -999999.;  // flct.c:287
@@ -55,6 +57,9 @@ struct SelectArgs {
   const double* f2;
   int nx, ny;
   double thresh;
+  // relative threshold: thresh is multiplied by max(max |f1|, max |f2|) read
+  // here (flct.c:594-629), or nullptr for an absolute threshold
+  const unsigned long long* absmax;
   int skip;       // 0: no skip
   int xoffset;    // q offset, applies to i (transp = 1)
   int yoffset;    // p offset, applies to j
@@ -64,6 +69,13 @@ struct SelectArgs {
   double* vy;
   double* vm;
 };
+
+// clears n counters (a kernel rather than a memset, so that graphs that
+// capture it replay correctly)
+__global__ void k_zero(unsigned int* __restrict__ p, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0u;
+}
 
 // threshold (flct.c:721-722) and skip (flct.c:734-757) predicates; pixels
 // that are not tracked get vx = vy = xmiss, vm = 0 (flct.c:918-930)
@@ -75,7 +87,7 @@ __global__ void k_select(SelectArgs a) {
   if (t < n) {
     const int i = (int)(t / a.ny), j = (int)(t - (long long)i * a.ny);
     const double fabsbar = 0.5 * (fabs(a.f1[t] + a.f2[t]));
-    const bool below = (fabsbar < a.thresh);
+    const bool below = (fabsbar < resolve_thresh(a.thresh, a.absmax));
     bool noskip = true;
     if (a.skip) noskip = (!((i - a.xoffset) % a.skip)) && (!((j - a.yoffset) % a.skip));
     tracked = (!below) && noskip;

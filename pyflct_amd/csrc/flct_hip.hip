@@ -1,6 +1,7 @@
 // C ABI of the gfx950 FLCT tracker and Fourier shifter (include/flct_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -33,6 +34,11 @@ struct flct_hip_ctx {
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   DevBuf f1, f2, vx, vy, vm, list, count, gw, filt, absmax, wtmp, wF, wtw1, wtw2, w3, w4, w5,
       hs1, hs2, mean1, mean2, flags, fcount, segtab, elist, ecount;
+  // batch: own scratch, table arena and counters, and the cached graph
+  DevBuf bscr[7], barena, bcount;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<unsigned char> gkey;  // pairs the cached graph was captured for
   int seg_n = 0;          // box size of the split tables in segtab
   double seg_sigma = 0.;  // window width of the split tables
   int bf16_kernel = 0;    // FLCT_HIP_BF16_KERNEL: 0 automatic, 1 per-pixel kernels only
@@ -124,6 +130,12 @@ extern "C" int flct_hip_destroy(flct_hip_ctx* c) {
                     &c->fcount, &c->segtab, &c->elist, &c->ecount};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
+  for (DevBuf& b : c->bscr)
+    if (b.p) (void)hipFree(b.p);
+  if (c->barena.p) (void)hipFree(c->barena.p);
+  if (c->bcount.p) (void)hipFree(c->bcount.p);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -202,38 +214,245 @@ extern "C" int flct_hip_check_opts(int nx, int ny, const flct_hip_opts* o, int* 
   return FLCT_HIP_OK;
 }
 
-// Split-BF16 form through the row-segment kernel: interior columns by
-// segments of one row, clamped edge columns by the per-pixel kernel.
-static int track_bf16_seg(flct_hip_ctx* c, const LaunchCtx& lc, const TrackArgs& ta, double sigma,
-                          double thresh, int skip, int xoffset, int yoffset) {
-  const int N = 2 * ta.nsize;
-  int rc;
-  hipStream_t s = c->stream;
-  if (c->seg_n != N || c->seg_sigma != sigma) {
-    std::vector<unsigned short> tab;
-    seg_tables(N, sigma, tab);
-    if ((rc = ensure(c->segtab, tab.size() * 2))) return rc;
-    HIPCHK(hipMemcpyAsync(c->segtab.p, tab.data(), tab.size() * 2, hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
-    c->seg_n = N;
-    c->seg_sigma = sigma;
+namespace {
+
+// Host plan of one track call: the validated options, the paths taken and
+// the host-built tables.  No device work.
+struct Plan {
+  int nx = 0, ny = 0;
+  flct_hip_opts o;
+  Resolved r;
+  int N = 0, M = 0;
+  bool relative = false;  // -t below 1 without 'a': times the image maxima (flct.c:594-629)
+  bool sparse = false;    // threshold or skip: a tracked-pixel list
+  bool seg = false;       // split-BF16 row-segment kernel for interior columns
+  int ixc = 0, iyc = 0;   // skip-fill lattice size
+  std::vector<double> gw, filt;
+};
+
+int make_plan(int nx, int ny, const flct_hip_opts* o, int bf16_kernel, Plan* pl) {
+  if (!o || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
+  if (o->deltat == 0.) return FLCT_HIP_E_ARG;
+  int rc = resolve(o, nx, ny, &pl->r);
+  if (rc) return rc;
+  if (o->precision != FLCT_HIP_FP64 && o->precision != FLCT_HIP_FP32 &&
+      o->precision != FLCT_HIP_SPLIT_BF16)
+    return FLCT_HIP_E_UNSUPPORTED;
+  pl->nx = nx;
+  pl->ny = ny;
+  pl->o = *o;
+  pl->N = pl->r.N;
+  pl->M = pl->N / 4;
+  if (pl->N % 4 != 0 || pl->M < 1 || pl->M > kTrackMaxM) return FLCT_HIP_E_UNSUPPORTED;
+  if ((long long)nx * ny > 0x7fffffffLL) return FLCT_HIP_E_UNSUPPORTED;
+  const int N = pl->N;
+  // window table gw[d + N - 1] = exp(-(sigminv d)^2) (flct.c:652-663, separable)
+  const double sigminv = 1. / o->sigma;
+  pl->gw.resize(2 * N - 1);
+  for (int d = -(N - 1); d <= N - 1; ++d) {
+    const double ax = sigminv * (double)d;
+    pl->gw[d + N - 1] = std::exp(-ax * ax);
   }
-  const long long nedge_max = (ta.list ? ta.nwork : (long long)ta.nx * (N - 1));
-  if ((rc = ensure(c->elist, (size_t)(nedge_max > 0 ? nedge_max : 1) * sizeof(int))) ||
-      (rc = ensure(c->ecount, sizeof(unsigned int))))
-    return rc;
-  long long nedge = 0;
-  if ((rc = launch_edge_list(lc, ta.list, ta.nwork, ta.nx, ta.ny, ta.nsize, (int*)c->elist.p,
-                             (unsigned int*)c->ecount.p, &nedge)))
-    return rc;
-  if (nedge > 0) {
-    TrackArgs te = ta;
-    te.list = (const int*)c->elist.p;
-    te.nwork = nedge;
-    if ((rc = dispatch_track_bf16(N, lc, te))) return rc;
+  // filter table over (k, l), k = 0..N/2, l = 0..N-1 (flct.c:1694-1715)
+  pl->filt.clear();
+  if (o->filter) {
+    std::vector<double> kx(N);
+    int n21 = (N / 2) - 1, inext;
+    for (int i = 0; i < n21 + 1; i++) kx[i] = (double)i;
+    inext = n21 + 1;
+    kx[inext] = (double)(N / 2);
+    inext++;
+    for (int i = inext; i < N; i++) kx[i] = -(double)(n21 - (i - inext));
+    const double kxroll = o->kr * kx[N / 2];
+    const double smxinv = 1. / kxroll;
+    pl->filt.resize((N / 2 + 1) * N);
+    for (int k = 0; k <= N / 2; ++k) {
+      const double argx = kx[k] * smxinv;
+      for (int l = 0; l < N; ++l) {
+        const double argy = kx[l] * smxinv;
+        pl->filt[k * N + l] = std::exp(-(argx * argx + argy * argy));
+      }
+    }
   }
-  return launch_track_seg(lc, ta, c->segtab.p, thresh, skip, xoffset, yoffset);
+  pl->relative = (o->thresh > 0.) && (o->thresh < 1.) && (!o->thresh_abs);
+  pl->sparse = (o->thresh > 0.) || pl->r.skipon;
+  pl->seg = (o->precision == FLCT_HIP_SPLIT_BF16) && !bf16_kernel && seg_supported(N, nx, ny, o->bias);
+  if (pl->r.skipon && o->interp) {
+    int ixc = 0, iyc = 0;
+    for (int i = 0; i < nx; i++)
+      if (!((i - pl->r.qoff) % o->skip)) ixc++;
+    for (int j = 0; j < ny; j++)
+      if (!((j - pl->r.poff) % o->skip)) iyc++;
+    // interpcc2d reads three lattice points on each side (flct.c:2294-2336)
+    if (ixc < 3 || iyc < 3) return FLCT_HIP_E_UNSUPPORTED;
+    pl->ixc = ixc;
+    pl->iyc = iyc;
+  }
+  return FLCT_HIP_OK;
 }
+
+// Device tables of one call.
+struct Tables {
+  const double* gw = nullptr;
+  const double* filt = nullptr;
+  const void* segtab = nullptr;
+};
+
+// Device scratch of one call, reused by the pairs of a batch in stream order.
+struct Scratch {
+  int* list = nullptr;   // tracked pixels (threshold or skip)
+  double* hs1 = nullptr;
+  double* hs2 = nullptr;
+  double* mean1 = nullptr;
+  double* mean2 = nullptr;
+  int* flags = nullptr;  // split-BF16 pixels for FP64 re-evaluation
+  int* elist = nullptr;  // edge-column pixels of the row-segment path
+};
+
+// Scratch bytes of one plan: list, hs1, hs2, mean1, mean2, flags, elist
+void scratch_sizes(const Plan& pl, size_t sz[7]) {
+  const long long npix = (long long)pl.nx * pl.ny;
+  const int N = pl.N, nsx = pl.ny - N + 1, nrx = pl.nx - N + 1;
+  sz[0] = pl.sparse ? npix * sizeof(int) : 0;
+  sz[1] = sz[2] = (size_t)pl.nx * nsx * 8;
+  sz[3] = sz[4] = (size_t)nrx * nsx * 8;
+  sz[5] = (pl.o.precision == FLCT_HIP_SPLIT_BF16) ? npix * sizeof(int) : 0;
+  sz[6] = pl.seg ? (pl.sparse ? npix : (long long)pl.nx * (N - 1)) * sizeof(int) : 0;
+}
+
+// Enqueues the tracking pipeline of one pair on lc.stream: no host
+// synchronisation and no memset, so a graph can capture it.
+// counts[0]: tracked-list length (threshold or skip); counts[1]: pixels
+// listed for FP64 re-evaluation; counts[2]: edge pixels; absmax: the two
+// image maxima.  ev (optional): events around the tracking kernels.
+int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const Scratch& sc,
+                  unsigned int* counts, unsigned long long* absmax, const double* f1,
+                  const double* f2, double* vx, double* vy, double* vm, hipEvent_t* ev) {
+  hipStream_t s = lc.stream;
+  const flct_hip_opts* o = &pl.o;
+  const int nx = pl.nx, ny = pl.ny, N = pl.N, M = pl.M;
+  const long long npix = (long long)nx * ny;
+  const bool bf16 = (o->precision == FLCT_HIP_SPLIT_BF16);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, counts, 4);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned int*>(absmax), 4);
+  // image maxima: relative threshold (flct.c:594-629) and the split-BF16 scaling
+  if (pl.relative || bf16) {
+    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, f1, npix, absmax);
+    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, f2, npix, absmax + 1);
+  }
+  const int xoffset = pl.r.qoff, yoffset = pl.r.poff;  // transp = 1 (flct.c:736-740)
+  if (pl.sparse) {
+    SelectArgs sa;
+    sa.f1 = f1;
+    sa.f2 = f2;
+    sa.nx = nx;
+    sa.ny = ny;
+    sa.thresh = o->thresh;
+    sa.absmax = pl.relative ? absmax : nullptr;
+    sa.skip = pl.r.skipon ? o->skip : 0;
+    sa.xoffset = xoffset;
+    sa.yoffset = yoffset;
+    sa.list = sc.list;
+    sa.count = counts;
+    sa.vx = vx;
+    sa.vy = vy;
+    sa.vm = vm;
+    hipLaunchKernelGGL(k_select, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, sa);
+  }
+  // box means by box origin (flct.c:815-827)
+  const long long nh = (long long)nx * (ny - N + 1), nm = (long long)(nx - N + 1) * (ny - N + 1);
+  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, f1, nx, ny, N, sc.hs1);
+  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, f2, nx, ny, N, sc.hs2);
+  hipLaunchKernelGGL(k_boxmean, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s,
+                     (const double*)sc.hs1, nx, ny, N, sc.mean1);
+  hipLaunchKernelGGL(k_boxmean, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s,
+                     (const double*)sc.hs2, nx, ny, N, sc.mean2);
+  HIPCHK(hipGetLastError());
+
+  TrackArgs ta;
+  ta.f1 = f1;
+  ta.f2 = f2;
+  ta.nx = nx;
+  ta.ny = ny;
+  ta.nsize = pl.r.nsize;
+  ta.sigminv = 1. / o->sigma;
+  ta.deltinv = 1. / o->deltat;
+  ta.deltas = o->deltas;
+  ta.gw = tb.gw;
+  ta.mean1 = sc.mean1;
+  ta.mean2 = sc.mean2;
+  ta.filt = tb.filt;
+  ta.list = pl.sparse ? sc.list : nullptr;
+  ta.nwork = npix;  // upper bound of the list length, for the grid
+  ta.nwork_dev = pl.sparse ? counts : nullptr;
+  ta.bias = o->bias ? 1 : 0;
+  ta.vx = vx;
+  ta.vy = vy;
+  ta.vm = vm;
+  ta.absmax = bf16 ? absmax : nullptr;
+  ta.flag_list = bf16 ? sc.flags : nullptr;
+  ta.flag_count = bf16 ? counts + 1 : nullptr;
+  if (ev) HIPCHK(hipEventRecord(ev[1], s));
+  int rc = FLCT_HIP_OK;
+  if (bf16) {
+    if (pl.seg) {
+      long long nedge = 0;
+      if ((rc = launch_edge_list(lc, ta.list, ta.nwork_dev, nx, ny, ta.nsize, sc.elist, counts + 2,
+                                 &nedge)))
+        return rc;
+      if (nedge != 0) {
+        TrackArgs te = ta;
+        te.list = sc.elist;
+        te.nwork = (nedge > 0) ? nedge : npix;
+        te.nwork_dev = (nedge > 0) ? nullptr : counts + 2;
+        if ((rc = dispatch_track_bf16(N, lc, te))) return rc;
+      }
+      if ((rc = launch_track_seg(lc, ta, tb.segtab, o->thresh, pl.relative ? 1 : 0,
+                                 pl.r.skipon ? o->skip : 0, xoffset, yoffset)))
+        return rc;
+    } else if ((rc = dispatch_track_bf16(N, lc, ta))) {
+      return rc;
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[4], s));
+    // FP64 re-evaluation of the flagged pixels (count read on the device)
+    TrackArgs tr = ta;
+    tr.list = sc.flags;
+    tr.nwork = npix;
+    tr.nwork_dev = counts + 1;
+    tr.absmax = nullptr;
+    tr.flag_list = nullptr;
+    tr.flag_count = nullptr;
+    rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, tr) : dispatch_track_f64_hi(M, lc, tr);
+  } else if (o->precision == FLCT_HIP_FP32) {
+    rc = (M <= 12) ? dispatch_track_f32_lo(M, lc, ta) : dispatch_track_f32_hi(M, lc, ta);
+  } else {
+    rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, ta) : dispatch_track_f64_hi(M, lc, ta);
+  }
+  if (rc) return rc;
+  if (ev) HIPCHK(hipEventRecord(ev[2], s));
+
+  // skip fill (flct.c:958-1120) and final reset (flct.c:1125-1133)
+  const unsigned pblocks = (unsigned)((npix + 255) / 256);
+  if (pl.r.skipon && o->interp) {
+    FillArgs fa;
+    fa.nx = nx;
+    fa.ny = ny;
+    fa.skip = o->skip;
+    fa.xoffset = xoffset;
+    fa.yoffset = yoffset;
+    fa.ixcount = pl.ixc;
+    fa.iycount = pl.iyc;
+    fa.vx = vx;
+    fa.vy = vy;
+    fa.vm = vm;
+    hipLaunchKernelGGL(k_fill, dim3(pblocks), dim3(256), 0, s, fa);
+  }
+  hipLaunchKernelGGL(k_finish, dim3(pblocks), dim3(256), 0, s, vx, vy, vm, npix);
+  HIPCHK(hipGetLastError());
+  return FLCT_HIP_OK;
+}
+
+}  // namespace
 
 static double ev_ms(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
@@ -241,234 +460,74 @@ static double ev_ms(hipEvent_t a, hipEvent_t b) {
   return (double)ms;
 }
 
+// Scratch buffers of the context sized for a plan
+static int ensure_scratch(flct_hip_ctx* c, const Plan& pl, Scratch* sc) {
+  size_t sz[7];
+  scratch_sizes(pl, sz);
+  DevBuf* b[7] = {&c->list, &c->hs1, &c->hs2, &c->mean1, &c->mean2, &c->flags, &c->elist};
+  for (int k = 0; k < 7; ++k)
+    if (int rc = ensure(*b[k], sz[k] ? sz[k] : 256)) return rc;
+  sc->list = (int*)c->list.p;
+  sc->hs1 = (double*)c->hs1.p;
+  sc->hs2 = (double*)c->hs2.p;
+  sc->mean1 = (double*)c->mean1.p;
+  sc->mean2 = (double*)c->mean2.p;
+  sc->flags = (int*)c->flags.p;
+  sc->elist = (int*)c->elist.p;
+  return FLCT_HIP_OK;
+}
+
 extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const double* f2, int nx,
                                      int ny, const flct_hip_opts* o, double* vx, double* vy,
                                      double* vm) {
   if (!c || !f1 || !f2 || !o || !vx || !vy || !vm || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
-  if (o->deltat == 0.) return FLCT_HIP_E_ARG;
-  Resolved r;
-  int rc = resolve(o, nx, ny, &r);
+  Plan pl;
+  int rc = make_plan(nx, ny, o, c->bf16_kernel, &pl);
   if (rc) return rc;
-  if (o->precision != FLCT_HIP_FP64 && o->precision != FLCT_HIP_FP32 &&
-      o->precision != FLCT_HIP_SPLIT_BF16)
-    return FLCT_HIP_E_UNSUPPORTED;
-  const int N = r.N, M = N / 4;
-  if (N % 4 != 0 || M < 1 || M > kTrackMaxM) return FLCT_HIP_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->device));
-  const long long npix = (long long)nx * ny;
-  if (npix > 0x7fffffffLL) return FLCT_HIP_E_UNSUPPORTED;
   hipStream_t s = c->stream;
   HIPCHK(hipEventRecord(c->ev[0], s));
-
-  // window table gw[d + N - 1] = exp(-(sigminv d)^2) (flct.c:652-663, separable)
-  const double sigminv = 1. / o->sigma;
-  std::vector<double> gw(2 * N - 1);
-  for (int d = -(N - 1); d <= N - 1; ++d) {
-    const double ax = sigminv * (double)d;
-    gw[d + N - 1] = std::exp(-ax * ax);
+  // tables
+  Tables tb;
+  if ((rc = ensure(c->gw, pl.gw.size() * 8))) return rc;
+  HIPCHK(hipMemcpyAsync(c->gw.p, pl.gw.data(), pl.gw.size() * 8, hipMemcpyHostToDevice, s));
+  tb.gw = (const double*)c->gw.p;
+  if (!pl.filt.empty()) {
+    if ((rc = ensure(c->filt, pl.filt.size() * 8))) return rc;
+    HIPCHK(hipMemcpyAsync(c->filt.p, pl.filt.data(), pl.filt.size() * 8, hipMemcpyHostToDevice, s));
+    tb.filt = (const double*)c->filt.p;
   }
-  if ((rc = ensure(c->gw, gw.size() * sizeof(double)))) return rc;
-  HIPCHK(hipMemcpyAsync(c->gw.p, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  // filter table over (k, l), k = 0..N/2, l = 0..N-1 (flct.c:1694-1715)
-  const double* d_filt = nullptr;
-  if (o->filter) {
-    std::vector<double> kx(N), filt((N / 2 + 1) * N);
-    {
-      int n21 = (N / 2) - 1, inext;
-      for (int i = 0; i < n21 + 1; i++) kx[i] = (double)i;
-      inext = n21 + 1;
-      kx[inext] = (double)(N / 2);
-      inext++;
-      for (int i = inext; i < N; i++) kx[i] = -(double)(n21 - (i - inext));
+  if (pl.seg) {
+    if (c->seg_n != pl.N || c->seg_sigma != o->sigma) {
+      std::vector<unsigned short> tab;
+      seg_tables(pl.N, o->sigma, tab);
+      if ((rc = ensure(c->segtab, tab.size() * 2))) return rc;
+      HIPCHK(hipMemcpyAsync(c->segtab.p, tab.data(), tab.size() * 2, hipMemcpyHostToDevice, s));
+      c->seg_n = pl.N;
+      c->seg_sigma = o->sigma;
     }
-    const double kxroll = o->kr * kx[N / 2];
-    const double smxinv = 1. / kxroll;
-    for (int k = 0; k <= N / 2; ++k) {
-      const double argx = kx[k] * smxinv;
-      for (int l = 0; l < N; ++l) {
-        const double argy = kx[l] * smxinv;
-        filt[k * N + l] = std::exp(-(argx * argx + argy * argy));
-      }
-    }
-    if ((rc = ensure(c->filt, filt.size() * sizeof(double)))) return rc;
-    HIPCHK(hipMemcpyAsync(c->filt.p, filt.data(), filt.size() * sizeof(double),
-                          hipMemcpyHostToDevice, s));
-    d_filt = (const double*)c->filt.p;
+    tb.segtab = c->segtab.p;
   }
-
-  // threshold: relative -> absolute (flct.c:594-629)
-  double thresh = o->thresh;
-  if ((thresh > 0.) && (thresh < 1.) && (!o->thresh_abs)) {
-    if ((rc = ensure(c->absmax, 2 * sizeof(unsigned long long)))) return rc;
-    HIPCHK(hipMemsetAsync(c->absmax.p, 0, 2 * sizeof(unsigned long long), s));
-    auto* am = (unsigned long long*)c->absmax.p;
-    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, f1, npix, am);
-    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, f2, npix, am + 1);
-    HIPCHK(hipGetLastError());
-    unsigned long long h[2];
-    HIPCHK(hipMemcpyAsync(h, am, sizeof(h), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    double f1max, f2max;
-    std::memcpy(&f1max, &h[0], 8);
-    std::memcpy(&f2max, &h[1], 8);
-    const double fmax = (f1max > f2max) ? f1max : f2max;
-    thresh *= fmax;
-  }
-
-  // tracked pixel selection
-  const bool sparse = (thresh > 0.) || r.skipon;
-  long long nwork = npix;
-  const int* d_list = nullptr;
-  const int xoffset = r.qoff, yoffset = r.poff;  // transp = 1 (flct.c:736-740)
-  if (sparse) {
-    if ((rc = ensure(c->list, npix * sizeof(int)))) return rc;
-    if ((rc = ensure(c->count, sizeof(unsigned int)))) return rc;
-    HIPCHK(hipMemsetAsync(c->count.p, 0, sizeof(unsigned int), s));
-    SelectArgs sa;
-    sa.f1 = f1;
-    sa.f2 = f2;
-    sa.nx = nx;
-    sa.ny = ny;
-    sa.thresh = thresh;
-    sa.skip = r.skipon ? o->skip : 0;
-    sa.xoffset = xoffset;
-    sa.yoffset = yoffset;
-    sa.list = (int*)c->list.p;
-    sa.count = (unsigned int*)c->count.p;
-    sa.vx = vx;
-    sa.vy = vy;
-    sa.vm = vm;
-    hipLaunchKernelGGL(k_select, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, sa);
-    HIPCHK(hipGetLastError());
-    unsigned int cnt = 0;
-    HIPCHK(hipMemcpyAsync(&cnt, c->count.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    nwork = cnt;
-    d_list = (const int*)c->list.p;
-  }
-
-  // box means by box origin (flct.c:815-827)
-  const int nsx = ny - N + 1, nrx = nx - N + 1;
-  if ((rc = ensure(c->hs1, (size_t)nx * nsx * 8)) || (rc = ensure(c->hs2, (size_t)nx * nsx * 8)) ||
-      (rc = ensure(c->mean1, (size_t)nrx * nsx * 8)) || (rc = ensure(c->mean2, (size_t)nrx * nsx * 8)))
+  Scratch sc;
+  if ((rc = ensure_scratch(c, pl, &sc))) return rc;
+  if ((rc = ensure(c->count, 4 * sizeof(unsigned int))) ||
+      (rc = ensure(c->absmax, 2 * sizeof(unsigned long long))))
     return rc;
-  {
-    const long long nh = (long long)nx * nsx, nm = (long long)nrx * nsx;
-    hipLaunchKernelGGL(k_rowsum, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, f1, nx, ny, N,
-                       (double*)c->hs1.p);
-    hipLaunchKernelGGL(k_rowsum, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, f2, nx, ny, N,
-                       (double*)c->hs2.p);
-    hipLaunchKernelGGL(k_boxmean, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s,
-                       (const double*)c->hs1.p, nx, ny, N, (double*)c->mean1.p);
-    hipLaunchKernelGGL(k_boxmean, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s,
-                       (const double*)c->hs2.p, nx, ny, N, (double*)c->mean2.p);
-    HIPCHK(hipGetLastError());
-  }
-
-  TrackArgs ta;
-  ta.f1 = f1;
-  ta.f2 = f2;
-  ta.nx = nx;
-  ta.ny = ny;
-  ta.nsize = r.nsize;
-  ta.sigminv = sigminv;
-  ta.deltinv = 1. / o->deltat;
-  ta.deltas = o->deltas;
-  ta.gw = (const double*)c->gw.p;
-  ta.mean1 = (const double*)c->mean1.p;
-  ta.mean2 = (const double*)c->mean2.p;
-  ta.filt = d_filt;
-  ta.list = d_list;
-  ta.nwork = nwork;
-  ta.bias = o->bias ? 1 : 0;
-  ta.vx = vx;
-  ta.vy = vy;
-  ta.vm = vm;
-  ta.absmax = nullptr;
-  ta.flag_list = nullptr;
-  ta.flag_count = nullptr;
-  c->refined = 0;
-  if (o->precision == FLCT_HIP_SPLIT_BF16 && nwork > 0) {
-    // pixels the BF16 surface cannot settle are listed for FP64 re-evaluation
-    if ((rc = ensure(c->flags, npix * sizeof(int))) || (rc = ensure(c->fcount, sizeof(unsigned int))))
-      return rc;
-    HIPCHK(hipMemsetAsync(c->fcount.p, 0, sizeof(unsigned int), s));
-    ta.flag_list = (int*)c->flags.p;
-    ta.flag_count = (unsigned int*)c->fcount.p;
-    // image maxima for the power-of-two input scaling of the BF16 form
-    if ((rc = ensure(c->absmax, 2 * sizeof(unsigned long long)))) return rc;
-    auto* am = (unsigned long long*)c->absmax.p;
-    HIPCHK(hipMemsetAsync(am, 0, 2 * sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, f1, npix, am);
-    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, f2, npix, am + 1);
-    HIPCHK(hipGetLastError());
-    ta.absmax = am;
-  }
-  HIPCHK(hipEventRecord(c->ev[1], s));
-  if (nwork > 0) {
-    LaunchCtx lc{c->stream, c->num_cu, c->lds_per_block};
-    if (o->precision == FLCT_HIP_SPLIT_BF16) {
-      if (!c->bf16_kernel && seg_supported(N, nx, ny, o->bias))
-        rc = track_bf16_seg(c, lc, ta, o->sigma, thresh, r.skipon ? o->skip : 0, xoffset, yoffset);
-      else
-        rc = dispatch_track_bf16(N, lc, ta);
-      if (!rc) {
-        HIPCHK(hipEventRecord(c->ev[4], s));
-        unsigned int nflag = 0;
-        HIPCHK(hipMemcpyAsync(&nflag, c->fcount.p, sizeof(nflag), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        c->refined = nflag;
-        if (nflag > 0) {
-          TrackArgs tb = ta;
-          tb.list = (const int*)c->flags.p;
-          tb.nwork = nflag;
-          tb.absmax = nullptr;
-          tb.flag_list = nullptr;
-          tb.flag_count = nullptr;
-          rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, tb) : dispatch_track_f64_hi(M, lc, tb);
-        }
-      }
-    }
-    else if (o->precision == FLCT_HIP_FP32)
-      rc = (M <= 12) ? dispatch_track_f32_lo(M, lc, ta) : dispatch_track_f32_hi(M, lc, ta);
-    else
-      rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, ta) : dispatch_track_f64_hi(M, lc, ta);
-    if (rc) return rc;
-  }
-  HIPCHK(hipEventRecord(c->ev[2], s));
-
-  // skip fill (flct.c:958-1120) and final reset (flct.c:1125-1133)
-  const unsigned pblocks = (unsigned)((npix + 255) / 256);
-  if (r.skipon && o->interp) {
-    FillArgs fa;
-    fa.nx = nx;
-    fa.ny = ny;
-    fa.skip = o->skip;
-    fa.xoffset = xoffset;
-    fa.yoffset = yoffset;
-    int ixc = 0, iyc = 0;
-    for (int i = 0; i < nx; i++)
-      if (!((i - xoffset) % o->skip)) ixc++;
-    for (int j = 0; j < ny; j++)
-      if (!((j - yoffset) % o->skip)) iyc++;
-    // interpcc2d reads three lattice points on each side (flct.c:2294-2336)
-    if (ixc < 3 || iyc < 3) return FLCT_HIP_E_UNSUPPORTED;
-    fa.ixcount = ixc;
-    fa.iycount = iyc;
-    fa.vx = vx;
-    fa.vy = vy;
-    fa.vm = vm;
-    hipLaunchKernelGGL(k_fill, dim3(pblocks), dim3(256), 0, s, fa);
-    HIPCHK(hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_finish, dim3(pblocks), dim3(256), 0, s, vx, vy, vm, npix);
-  HIPCHK(hipGetLastError());
+  LaunchCtx lc{s, c->num_cu, c->lds_per_block};
+  unsigned int* counts = (unsigned int*)c->count.p;
+  if ((rc = enqueue_track(lc, pl, tb, sc, counts, (unsigned long long*)c->absmax.p, f1, f2, vx, vy,
+                          vm, c->ev)))
+    return rc;
   HIPCHK(hipEventRecord(c->ev[3], s));
+  unsigned int hc[4];
+  HIPCHK(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  c->last_tracked = nwork;
+  const bool bf16 = (o->precision == FLCT_HIP_SPLIT_BF16);
+  c->last_tracked = pl.sparse ? (long long)hc[0] : (long long)nx * ny;
+  c->refined = bf16 ? (long long)hc[1] : 0;
   c->last_kernel_ms = ev_ms(c->ev[1], c->ev[2]);
   c->last_total_ms = ev_ms(c->ev[0], c->ev[3]);
-  if (o->precision == FLCT_HIP_SPLIT_BF16 && nwork > 0) {
+  if (bf16) {
     c->last_main_ms = ev_ms(c->ev[1], c->ev[4]);
     c->last_refine_ms = ev_ms(c->ev[4], c->ev[2]);
   } else {
@@ -496,6 +555,147 @@ extern "C" int flct_hip_track(flct_hip_ctx* c, const double* f1, const double* f
   HIPCHK(hipMemcpyAsync(vy, c->vy.p, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(vm, c->vm.p, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return FLCT_HIP_OK;
+}
+
+// ------------------------------------------------------------------ batch
+
+static void drop_graph(flct_hip_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->gkey.clear();
+}
+
+// ensure() for buffers a cached graph may reference: a reallocation drops it
+static int ensure_b(flct_hip_ctx* c, DevBuf& b, size_t bytes) {
+  if (b.p && b.bytes >= bytes) return FLCT_HIP_OK;
+  drop_graph(c);
+  return ensure(b, bytes);
+}
+
+extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int npairs) {
+  if (!c || npairs < 0 || (npairs > 0 && !pairs)) return FLCT_HIP_E_ARG;
+  if (npairs == 0) return FLCT_HIP_OK;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  // plans of every pair; nothing runs when one is invalid
+  std::vector<Plan> pl(npairs);
+  int first_err = FLCT_HIP_OK;
+  for (int k = 0; k < npairs; ++k) {
+    flct_hip_pair& q = pairs[k];
+    int rc = (!q.f1 || !q.f2 || !q.vx || !q.vy || !q.vm) ? FLCT_HIP_E_ARG
+                                                          : make_plan(q.nx, q.ny, &q.opts, c->bf16_kernel, &pl[k]);
+    q.status = rc;
+    q.tracked = 0;
+    q.refined = 0;
+    if (rc && !first_err) first_err = rc;
+  }
+  if (first_err) return first_err;
+  // table arena: gw, filt, split tables (one copy per distinct box size and sigma)
+  std::vector<unsigned char> arena;
+  auto put = [&](const void* src, size_t bytes) {
+    const size_t off = (arena.size() + 255) / 256 * 256;
+    arena.resize(off + bytes);
+    std::memcpy(arena.data() + off, src, bytes);
+    return off;
+  };
+  std::vector<size_t> o_gw(npairs), o_filt(npairs, (size_t)-1), o_seg(npairs, (size_t)-1);
+  std::vector<std::pair<std::pair<int, double>, size_t>> segs;
+  for (int k = 0; k < npairs; ++k) {
+    o_gw[k] = put(pl[k].gw.data(), pl[k].gw.size() * 8);
+    if (!pl[k].filt.empty()) o_filt[k] = put(pl[k].filt.data(), pl[k].filt.size() * 8);
+    if (pl[k].seg) {
+      const std::pair<int, double> key(pl[k].N, pl[k].o.sigma);
+      for (auto& e : segs)
+        if (e.first == key) o_seg[k] = e.second;
+      if (o_seg[k] == (size_t)-1) {
+        std::vector<unsigned short> tab;
+        seg_tables(pl[k].N, pl[k].o.sigma, tab);
+        o_seg[k] = put(tab.data(), tab.size() * 2);
+        segs.push_back({key, o_seg[k]});
+      }
+    }
+  }
+  // scratch for the largest pair, counters per pair
+  size_t need[7] = {256, 256, 256, 256, 256, 256, 256};
+  for (int k = 0; k < npairs; ++k) {
+    size_t sz[7];
+    scratch_sizes(pl[k], sz);
+    for (int i = 0; i < 7; ++i) need[i] = std::max(need[i], sz[i]);
+  }
+  int rc;
+  for (int i = 0; i < 7; ++i)
+    if ((rc = ensure_b(c, c->bscr[i], need[i]))) return rc;
+  if ((rc = ensure_b(c, c->barena, arena.size())) ||
+      (rc = ensure_b(c, c->bcount, (size_t)npairs * 32)))
+    return rc;
+  HIPCHK(hipMemcpyAsync(c->barena.p, arena.data(), arena.size(), hipMemcpyHostToDevice, s));
+  // the graph is reused while the pairs are the same
+  std::vector<unsigned char> key((size_t)npairs * sizeof(flct_hip_pair));
+  for (int k = 0; k < npairs; ++k) {
+    flct_hip_pair q = pairs[k];
+    q.tracked = 0;
+    q.refined = 0;
+    q.status = 0;
+    std::memcpy(key.data() + (size_t)k * sizeof(q), &q, sizeof(q));
+  }
+  if (!c->gexec || key != c->gkey) {
+    drop_graph(c);
+    Scratch sc;
+    sc.list = (int*)c->bscr[0].p;
+    sc.hs1 = (double*)c->bscr[1].p;
+    sc.hs2 = (double*)c->bscr[2].p;
+    sc.mean1 = (double*)c->bscr[3].p;
+    sc.mean2 = (double*)c->bscr[4].p;
+    sc.flags = (int*)c->bscr[5].p;
+    sc.elist = (int*)c->bscr[6].p;
+    unsigned char* base = (unsigned char*)c->barena.p;
+    LaunchCtx lc{s, c->num_cu, c->lds_per_block};
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    int erc = FLCT_HIP_OK;
+    for (int k = 0; k < npairs && !erc; ++k) {
+      Tables tb;
+      tb.gw = (const double*)(base + o_gw[k]);
+      tb.filt = (o_filt[k] != (size_t)-1) ? (const double*)(base + o_filt[k]) : nullptr;
+      tb.segtab = (o_seg[k] != (size_t)-1) ? (const void*)(base + o_seg[k]) : nullptr;
+      unsigned char* cnt = (unsigned char*)c->bcount.p + (size_t)k * 32;
+      erc = enqueue_track(lc, pl[k], tb, sc, (unsigned int*)cnt, (unsigned long long*)(cnt + 16),
+                          pairs[k].f1, pairs[k].f2, pairs[k].vx, pairs[k].vy, pairs[k].vm, nullptr);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    if (erc) {
+      if (g) (void)hipGraphDestroy(g);
+      return erc;
+    }
+    if (ec != hipSuccess || !g) return FLCT_HIP_E_HIP;
+    c->graph = g;
+    if (hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+      drop_graph(c);
+      return FLCT_HIP_E_HIP;
+    }
+    c->gkey = key;
+  }
+  HIPCHK(hipEventRecord(c->ev[0], s));
+  HIPCHK(hipGraphLaunch(c->gexec, s));
+  HIPCHK(hipEventRecord(c->ev[3], s));
+  std::vector<unsigned int> hc((size_t)npairs * 8);
+  HIPCHK(hipMemcpyAsync(hc.data(), c->bcount.p, hc.size() * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  long long tracked = 0, refined = 0;
+  for (int k = 0; k < npairs; ++k) {
+    const unsigned int* ck = hc.data() + (size_t)k * 8;
+    pairs[k].tracked = pl[k].sparse ? (long long)ck[0] : (long long)pl[k].nx * pl[k].ny;
+    pairs[k].refined = (pl[k].o.precision == FLCT_HIP_SPLIT_BF16) ? (long long)ck[1] : 0;
+    tracked += pairs[k].tracked;
+    refined += pairs[k].refined;
+  }
+  c->last_tracked = tracked;
+  c->refined = refined;
+  c->last_kernel_ms = c->last_total_ms = c->last_main_ms = ev_ms(c->ev[0], c->ev[3]);
+  c->last_refine_ms = 0.;
   return FLCT_HIP_OK;
 }
 

@@ -171,37 +171,34 @@ void seg_tables(int N, double sigma, std::vector<unsigned short>& tab) {
       }
 }
 
-int launch_edge_list(const LaunchCtx& c, const int* list, long long nlist, int nx, int ny,
+int launch_edge_list(const LaunchCtx& c, const int* list, const unsigned int* nlist, int nx, int ny,
                      int nsize, int* out, unsigned int* count, long long* nedge) {
   const int jlo = nsize - 1, jhi = ny - nsize - 1;
   if (list) {
-    if (hipMemsetAsync(count, 0, sizeof(unsigned int), c.stream) != hipSuccess) return FLCT_HIP_E_HIP;
-    if (nlist > 0)
-      hipLaunchKernelGGL(k_edge_list, dim3((unsigned)((nlist + 255) / 256)), dim3(256), 0, c.stream,
-                         list, nlist, nx, ny, jlo, jhi, out, count);
-    unsigned int h = 0;
-    if (hipMemcpyAsync(&h, count, sizeof(h), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
-        hipStreamSynchronize(c.stream) != hipSuccess)
-      return FLCT_HIP_E_HIP;
-    *nedge = h;
+    // count must be zero on entry; the per-pixel kernel reads it on the device
+    hipLaunchKernelGGL(k_edge_list, dim3(1024), dim3(256), 0, c.stream, list, nlist, nx, ny, jlo, jhi,
+                       out, count);
+    *nedge = -1;
   } else {
     const long long ne = (long long)nx * (jlo + (ny - 1 - jhi));
     if (ne > 0)
       hipLaunchKernelGGL(k_edge_list, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c.stream,
-                         (const int*)nullptr, 0LL, nx, ny, jlo, jhi, out, count);
+                         (const int*)nullptr, (const unsigned int*)nullptr, nx, ny, jlo, jhi, out,
+                         count);
     *nedge = ne;
   }
   return hipGetLastError() == hipSuccess ? FLCT_HIP_OK : FLCT_HIP_E_HIP;
 }
 
 int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, double thresh,
-                     int skip, int xoffset, int yoffset) {
+                     int thresh_rel, int skip, int xoffset, int yoffset) {
   using C = SegCfg;
   const int N = 2 * a.nsize;
   SegArgs s;
   s.t = a;
   s.tab = reinterpret_cast<const __bf16*>(tab);
   s.thresh = thresh;
+  s.thresh_rel = thresh_rel;
   s.skip = skip;
   s.xoffset = xoffset;
   s.yoffset = yoffset;

@@ -40,7 +40,8 @@ struct TrackArgs {
   const double* mean2;
   const double* filt;   // (N/2+1) x N filter values, or nullptr
   const int* list;      // tracked pixel list, or nullptr for all pixels in order
-  long long nwork;
+  long long nwork;      // list length (or pixel count without a list)
+  const unsigned int* nwork_dev;  // list length in device memory (overrides nwork), or nullptr
   const unsigned long long* absmax;  // max |f1|, max |f2| as double bits, or nullptr
   int* flag_list;                    // pixels to re-evaluate in FP64, or nullptr
   unsigned int* flag_count;
@@ -97,6 +98,16 @@ __device__ __forceinline__ int zidx(int k, int c) {
 template <int M>
 __device__ __forceinline__ int park_col(int l) {
   return 4 * (l % M) + brev2(l / M);
+}
+
+// absolute threshold: thresh, or thresh * max(f1max, f2max) for a relative one
+// (flct.c:594-629); absmax holds the two maxima as double bits
+__device__ __forceinline__ double resolve_thresh(double thresh, const unsigned long long* absmax) {
+  if (!absmax) return thresh;
+  const double f1max = __longlong_as_double((long long)absmax[0]);
+  const double f2max = __longlong_as_double((long long)absmax[1]);
+  const double fmax = (f1max > f2max) ? f1max : f2max;
+  return thresh * fmax;
 }
 
 __device__ __forceinline__ unsigned long long dbits(double v) {
@@ -175,7 +186,7 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
   } else {
     first = (long long)blockIdx.x * Cfg::P + slot;
     stride = (long long)gridDim.x * Cfg::P;
-    count = a.nwork;
+    count = a.nwork_dev ? (long long)*a.nwork_dev : a.nwork;
   }
   const long long iters = (count + stride - 1) / stride;
   const int full_strips = ny / kStrip;

@@ -48,14 +48,16 @@ int dispatch_track_bf16(int N, const LaunchCtx& c, const TrackArgs& a);
 
 // split-BF16 row-segment form: boxes N = 36..64 with N % 4 == 0, interior
 // columns, no bias pass.  seg_tables fills the split DFT tables for box N and
-// window width sigma (bf16 bits, copied to the device by the caller); launch_edge_list lists the tracked pixels of the clamped edge
-// columns (from a tracked-pixel list, or every edge pixel when list is null)
-// and returns their number through *nedge.
+// window width sigma (bf16 bits, copied to the device by the caller).
+// launch_edge_list lists the tracked pixels of the clamped edge columns: from
+// a tracked-pixel list of device length *nlist into out / *count (zero on
+// entry; *nedge = -1, the count stays on the device), or every edge pixel
+// when list is null (*nedge = their number).
 bool seg_supported(int N, int nx, int ny, int bias);
 void seg_tables(int N, double sigma, std::vector<unsigned short>& tab);
-int launch_edge_list(const LaunchCtx& c, const int* list, long long nlist, int nx, int ny,
+int launch_edge_list(const LaunchCtx& c, const int* list, const unsigned int* nlist, int nx, int ny,
                      int nsize, int* out, unsigned int* count, long long* nedge);
 int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, double thresh,
-                     int skip, int xoffset, int yoffset);
+                     int thresh_rel, int skip, int xoffset, int yoffset);
 
 }  // namespace flct

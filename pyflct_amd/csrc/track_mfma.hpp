@@ -163,12 +163,13 @@ __global__ void __launch_bounds__((MfmaCfg<NP>::kThreads)) k_track_mfma(TrackArg
   const float norm = (float)((1. / ((double)N * N)) * (1. / ((double)N * N)));
   const long long nslots = (long long)gridDim.x * C::PIX;
   const long long first = (long long)blockIdx.x * C::PIX + ps;
-  const long long iters = (a.nwork + nslots - 1) / nslots;
+  const long long nwork = (a.list && a.nwork_dev) ? (long long)*a.nwork_dev : a.nwork;
+  const long long iters = (nwork + nslots - 1) / nslots;
   const int npass = a.bias ? 2 : 1;
 
   for (long long it = 0; it < iters; ++it) {
     const long long w = first + it * nslots;
-    const bool active = w < a.nwork;
+    const bool active = w < nwork;
     int i = 0, j = 0, imin = 0, jmin = 0;
     double f1bar = 0., f2bar = 0.;
     if (active) {

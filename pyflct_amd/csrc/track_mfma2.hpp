@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(R2Cfg<PIX>::kThreads, 2) k_track_r2(TrackArgs 
   } else {
     first = blockIdx.x * PIX + ps;
     stride = gridDim.x * PIX;
-    count = (unsigned)a.nwork;
+    count = a.nwork_dev ? *a.nwork_dev : (unsigned)a.nwork;
   }
   const unsigned iters = (count + stride - 1) / stride;
   const int full_strips = ny / kStrip;

@@ -50,7 +50,8 @@ namespace flct {
 struct SegArgs {
   TrackArgs t;           // images, box size, means, outputs, filter, flags, scales
   const __bf16* tab;     // split tables, SegCfg layout (global)
-  double thresh;         // absolute threshold (flct.c:721-722), 0: off
+  double thresh;         // threshold (flct.c:721-722), 0: off
+  int thresh_rel;        // thresh is relative: times max(max |f1|, max |f2|) from t.absmax
   int skip, xoffset, yoffset;  // flct.c:734-757 (transp = 1)
   int jlo;               // first interior column, nsize - 1
   int width;             // interior columns, ny - N + 1
@@ -200,7 +201,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         const int j = jb0 + tid;
         const long long t = (long long)i * ny + j;
         const double fabsbar = 0.5 * (fabs(a.f1[t] + a.f2[t]));
-        const bool below = (fabsbar < A.thresh);
+        const bool below = (fabsbar < resolve_thresh(A.thresh, A.thresh_rel ? a.absmax : nullptr));
         bool noskip = true;
         if (A.skip) noskip = (!((i - A.xoffset) % A.skip)) && (!((j - A.yoffset) % A.skip));
         tr = (!below) && noskip;
@@ -698,26 +699,33 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
 
 // Edge pixels for the per-pixel kernel: tracked pixels whose box is clamped
 // at the left or right image edge (columns outside [jlo, jhi]).  With a
-// tracked-pixel list (threshold or skip) the list is filtered; otherwise the
-// edge columns of every row are enumerated.
-__global__ void k_edge_list(const int* __restrict__ list, long long nlist, int nx, int ny, int jlo,
-                            int jhi, int* __restrict__ out, unsigned int* __restrict__ count) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// tracked-pixel list (threshold or skip; its length in *nlist) the list is
+// filtered into out / *count by a grid-stride loop; otherwise the edge
+// columns of every row are enumerated.
+__global__ void k_edge_list(const int* __restrict__ list, const unsigned int* __restrict__ nlist,
+                            int nx, int ny, int jlo, int jhi, int* __restrict__ out,
+                            unsigned int* __restrict__ count) {
   if (list) {
-    bool e = false;
-    int p = 0;
-    if (t < nlist) {
-      p = list[t];
-      const int j = p % ny;
-      e = (j < jlo) || (j > jhi);
+    const long long n = (long long)*nlist;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t0 = (long long)blockIdx.x * blockDim.x; t0 < n; t0 += stride) {
+      const long long t = t0 + threadIdx.x;
+      bool e = false;
+      int p = 0;
+      if (t < n) {
+        p = list[t];
+        const int j = p % ny;
+        e = (j < jlo) || (j > jhi);
+      }
+      const unsigned long long mask = __ballot(e);
+      const int ln = threadIdx.x & 63;
+      unsigned int base = 0;
+      if (ln == 0 && mask) base = atomicAdd(count, (unsigned)__popcll(mask));
+      base = __shfl(base, 0);
+      if (e) out[base + __popcll(mask & ((1ull << ln) - 1ull))] = p;
     }
-    const unsigned long long mask = __ballot(e);
-    const int ln = threadIdx.x & 63;
-    unsigned int base = 0;
-    if (ln == 0 && mask) base = atomicAdd(count, (unsigned)__popcll(mask));
-    base = __shfl(base, 0);
-    if (e) out[base + __popcll(mask & ((1ull << ln) - 1ull))] = p;
   } else {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int we = jlo + (ny - 1 - jhi);
     if (t < (long long)nx * we) {
       const int r = (int)(t / we), c = (int)(t - (long long)r * we);

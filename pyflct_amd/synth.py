@@ -147,6 +147,17 @@ def cfg3_inputs(seed=None):
     return img, delx, dely
 
 
+def uneven_inputs(n, seed):
+    """One pair of the uneven batch: an n^2 magnetogram (generator of the
+    512^2 case) and a smooth random velocity field (8 modes, wavelengths
+    64-512 px, max |v| 1.5 px); image 2 is warp(image 1, field), computed by
+    the caller's warp."""
+    rng = np.random.default_rng(seed)
+    img = magnetogram(n, rng)
+    delx, dely = fourier_mode_field(n, rng)
+    return img, delx, dely
+
+
 def cfg4(n=4096, seed=None, radius=None):
     """Full-disk-like pair: disk of radius 1900 px (scaled with n), Laplace
     scale 30 G plus 2000 bipolar active-region blobs up to +-3000 G inside,
