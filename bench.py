@@ -6,6 +6,7 @@ box means, the tracker kernel over all 16,777,216 pixels, and the final
 reset.  Inputs are in HBM before the timed region.  Prints one JSON line.
 """
 import argparse
+import collections
 import json
 import math
 import os
@@ -91,6 +92,10 @@ def main():
     ap.add_argument("--forms", default="fp64,fp32,split_bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--workload", default="full-disk", choices=["full-disk", "uneven-batch"],
+                    help="full-disk: the 4096^2 sigma=15 pair (default); uneven-batch: 64 pairs "
+                         "of sizes 256..2048 and sigma 4..20 in one graph launch")
+    ap.add_argument("--pairs", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not supported "
@@ -99,6 +104,9 @@ def main():
 
     from pyflct_amd import api, synth
     from pyflct_amd._lib import FlctHipError
+
+    if args.workload == "uneven-batch":
+        return uneven_batch(args)
 
     log(f"bench: generating the {args.size}^2 pair")
     f1, f2, vtrue = synth.cfg4(n=args.size)
@@ -229,6 +237,93 @@ def main():
     }
     for k in d.values():
         ctx.free(k)
+    print(json.dumps(out))
+
+
+def uneven_batch(args):
+    """64 image pairs, sizes uniform in {256, 512, 1024, 2048}^2, sigma
+    uniform in 4-20, each a magnetogram warped by its own smooth random
+    velocity field (8 modes, wavelengths 64-512 px, max |v| 1.5 px).  All
+    pairs are tracked by one flct_hip_track_batch call (one graph launch) in
+    FP64, FP32 and split-BF16; FP32 and split-BF16 are compared with FP64 and
+    image 1 is warped by the split-BF16 velocities to report the residual
+    against image 2."""
+    from pyflct_amd import api, synth
+    rng = np.random.default_rng(20261207)
+    ctx = api.Context(0)
+    pairs = []
+    total_px = 0
+    log(f"bench: generating {args.pairs} pairs")
+    for k in range(args.pairs):
+        n = int(rng.choice([256, 512, 1024, 2048]))
+        sigma = float(rng.uniform(4.0, 20.0))
+        f1, dx, dy = synth.uneven_inputs(n, seed=int(rng.integers(1 << 30)))
+        d = {key: ctx.malloc(f1.nbytes) for key in ("f1", "f2", "dx", "dy")}
+        ctx.upload(d["f1"], f1)
+        ctx.upload(d["dx"], dx)
+        ctx.upload(d["dy"], dy)
+        ctx.warp_device(d["f1"], d["dx"], d["dy"], d["f2"], n, n)
+        for form in ("fp64", "fp32", "split_bf16"):
+            for o in ("vx", "vy", "vm"):
+                d[f"{form}_{o}"] = ctx.malloc(f1.nbytes)
+        pairs.append((n, sigma, d))
+        total_px += n * n
+    log(f"bench: {total_px / 1e6:.1f} Mpx in {len(pairs)} pairs; "
+        f"sizes {sorted(collections.Counter(p[0] for p in pairs).items())}")
+    res = {}
+    for form in ("fp64", "fp32", "split_bf16"):
+        desc = [(d["f1"], d["f2"], d[f"{form}_vx"], d[f"{form}_vy"], d[f"{form}_vm"], n, n,
+                 ctx.make_opts(1.0, 1.0, sigma, precision=form)) for n, sigma, d in pairs]
+        for _ in range(max(1, args.warmup)):
+            ctx.track_batch_device(desc)  # the first call captures the graph
+        ms = []
+        for _ in range(args.steps):
+            st = ctx.track_batch_device(desc)
+            ms.append(ctx.last_stats()["kernel_ms"])
+        tracked = sum(x["tracked"] for x in st)
+        res[form] = {"ms": float(np.mean(ms)), "px_per_s": tracked / (np.mean(ms) * 1e-3),
+                     "tracked": tracked, "refined": sum(x["refined"] for x in st)}
+        log(f"bench: {form}: {np.mean(ms):.1f} ms per batch, {res[form]['px_per_s'] / 1e6:.1f} Mpx/s")
+    # errors against FP64 and the warp residual of the split-BF16 velocities
+    sq = {"fp32": 0.0, "split_bf16": 0.0}
+    mx = {"fp32": 0.0, "split_bf16": 0.0}
+    rres, rbase = [], []
+    for n, sigma, d in pairs:
+        ref = [ctx.download(np.empty((n, n)), d[f"fp64_{o}"]) for o in ("vx", "vy")]
+        for form in sq:
+            g = [ctx.download(np.empty((n, n)), d[f"{form}_{o}"]) for o in ("vx", "vy")]
+            e = np.hypot(g[0] - ref[0], g[1] - ref[1])
+            sq[form] += float(np.sum(e ** 2))
+            mx[form] = max(mx[form], float(e.max()))
+        ctx.warp_device(d["f1"], d["split_bf16_vx"], d["split_bf16_vy"], d["dx"], n, n)
+        w = ctx.download(np.empty((n, n)), d["dx"])
+        a = ctx.download(np.empty((n, n)), d["f1"])
+        b = ctx.download(np.empty((n, n)), d["f2"])
+        m = slice(n // 8, n - n // 8)
+        rres.append(np.sqrt(np.mean((w[m, m] - b[m, m]) ** 2)))
+        rbase.append(np.sqrt(np.mean((a[m, m] - b[m, m]) ** 2)))
+    for n, sigma, d in pairs:
+        for p_ in d.values():
+            ctx.free(p_)
+    hr = res["split_bf16"]
+    out = {
+        "metric": "uneven batch tracked px/s (64 pairs, 256..2048, sigma 4..20), split-BF16",
+        "value": round(hr["px_per_s"], 1),
+        "unit": "tracked px/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(hr["ms"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3+f32acc",
+        "data": "synthetic (PCG64 seed 20261207: magnetograms warped by random smooth fields)",
+        "config": {"workload": "uneven batch of 64 pairs in one graph launch", "pairs": len(pairs),
+                   "total_px": total_px},
+        "forms": {f: {"px_per_s": round(r["px_per_s"], 1), "ms_per_batch": round(r["ms"], 3),
+                      "refined_px": r["refined"]} for f, r in res.items()},
+        "rms_vel_err_vs_fp64": {f: float(np.sqrt(sq[f] / total_px)) for f in sq},
+        "max_vel_err_vs_fp64": mx,
+        "warp_residual_rms": {"split_bf16_velocities": float(np.sqrt(np.mean(np.square(rres)))),
+                              "unwarped": float(np.sqrt(np.mean(np.square(rbase))))},
+    }
     print(json.dumps(out))
 
 

@@ -233,6 +233,11 @@ class Context:
                                        ny, int(bool(transp)), 1), "flct_hip_warp")
         return out
 
+    def warp_device(self, d_arr, d_delx, d_dely, d_out, nx, ny, transp=True):
+        """warp_frac2d on device arrays (flct_hip_warp_device)."""
+        check(_lib.lib().flct_hip_warp_device(self._h, d_arr, d_delx, d_dely, d_out, int(nx),
+                                              int(ny), int(bool(transp))), "flct_hip_warp_device")
+
     def shift(self, arr, delx, dely, transp=True):
         arr = f64c(arr)
         nx, ny = arr.shape
