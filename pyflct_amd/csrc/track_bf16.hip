@@ -1,7 +1,7 @@
-// Split-BF16 MFMA tracker launches.  Boxes N = 36..64 with N % 4 == 0 use
-// the row-segment kernel for interior columns (no bias pass) and the radix-2
-// per-pixel kernel for the rest; other sizes use the dense DFT kernel padded
-// to NP = 32, 64 or 96.
+// Split-BF16 MFMA tracker launches.  Boxes N = 8..64 with N % 4 == 0 use the
+// row-segment kernel for interior columns (no bias pass) and a per-pixel
+// kernel for the edge columns: the radix-2 kernel for 36..64, the dense DFT
+// kernel padded to NP = 32, 64 or 96 otherwise.
 #include <cmath>
 #include <cstring>
 
@@ -61,7 +61,7 @@ int dispatch_track_bf16(int N, const LaunchCtx& c, const TrackArgs& a) {
 
 bool seg_supported(int N, int nx, int ny, int bias) {
   (void)nx;
-  return !bias && N >= 36 && N <= 64 && N % 4 == 0 && ny - N + 1 >= 1;
+  return !bias && N >= 8 && N <= 64 && N % 4 == 0 && ny - N + 1 >= 1;
 }
 
 // round to nearest even BF16
@@ -210,12 +210,15 @@ int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, do
   s.nseg = (int)nseg;
   const size_t lds = C::kLdsBytes;
   if (lds > c.lds_per_block) return FLCT_HIP_E_UNSUPPORTED;
-  (void)hipFuncSetAttribute((const void*)k_track_seg, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds);
+  const void* kern = (N <= 32) ? (const void*)k_track_seg<1> : (const void*)k_track_seg<2>;
+  (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   long long grid = c.num_cu;
   if (grid > nseg) grid = nseg;
   if (grid < 1) return FLCT_HIP_OK;
-  hipLaunchKernelGGL(k_track_seg, dim3((unsigned)grid), dim3(C::kThreads), lds, c.stream, s);
+  if (N <= 32)
+    hipLaunchKernelGGL(k_track_seg<1>, dim3((unsigned)grid), dim3(C::kThreads), lds, c.stream, s);
+  else
+    hipLaunchKernelGGL(k_track_seg<2>, dim3((unsigned)grid), dim3(C::kThreads), lds, c.stream, s);
   return hipGetLastError() == hipSuccess ? FLCT_HIP_OK : FLCT_HIP_E_HIP;
 }
 

@@ -46,7 +46,7 @@ int dispatch_track_f32_hi(int M, const LaunchCtx& c, const TrackArgs& a);
 // split-BF16 MFMA form, box size N = 4..96
 int dispatch_track_bf16(int N, const LaunchCtx& c, const TrackArgs& a);
 
-// split-BF16 row-segment form: boxes N = 36..64 with N % 4 == 0, interior
+// split-BF16 row-segment form: boxes N = 8..64 with N % 4 == 0, interior
 // columns, no bias pass.  seg_tables fills the split DFT tables for box N and
 // window width sigma (bf16 bits, copied to the device by the caller).
 // launch_edge_list lists the tracked pixels of the clamped edge columns: from

@@ -1,4 +1,4 @@
-// Split-BF16 MFMA tracker, row-segment form, for boxes N = 36..64 with
+// Split-BF16 MFMA tracker, row-segment form, for boxes N = 8..64 with
 // N % 4 == 0 and columns whose box is not clamped at the left or right edge.
 //
 // The windowed sub-image of pixel (i, j) is g(x, c) = Wr(x) Wc(c) (f(imin + x,
@@ -30,6 +30,10 @@
 //            Q1 = P(l) - P(l+Nh), the twiddle folded into the q = 1 table.
 //   inverse 2 (k -> ii, Hermitian half sum, radix-2 on ii):
 //            c(ii, jj) = Ce + Co, c(ii + Nh, jj) = Ce - Co over even / odd k.
+//
+// TL is the number of 16-wide tiles along k, l and the outputs: 2 for boxes
+// 36..64, 1 for boxes up to 32 (Nh <= 16), where the second tiles would be
+// padding.
 //
 // One wave tracks one pixel at a time.  Stage-2 outputs are rows l for the
 // lane's column k, which is exactly the A operand (row k, 8 contraction
@@ -130,6 +134,7 @@ __device__ __forceinline__ SegArgsPtr seg_args() {
   return p;
 }
 
+template <int TL>
 __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
   using C = SegCfg;
   constexpr int RS = C::RS, PL = C::kPlane, RW = C::RW, RK = C::RK, TP = C::kTPlane;
@@ -274,7 +279,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           Ah[s] = u4_to_bf8(hv);
           Al[s] = u4_to_bf8(lv);
         });
-        static_for<2>([&](auto kt) {
+        static_for<TL>([&](auto kt) {
           const int k = 16 * kt + n1;
           f32x4 tre = z4, tim = z4;
           static_for<2>([&](auto s) {
@@ -305,7 +310,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
     // sc_X Wr^(k) for the lane's k (T column 127: parity 1, uh 63)
     float2 wrh[2][2];  // [X][kt]
     static_for<2>([&](auto X) {
-      static_for<2>([&](auto kt) {
+      static_for<TL>([&](auto kt) {
         const __bf16* pl = TT + (X * 2 + 1) * TP + (16 * kt + n0) * RK + 63;
         const float re = (float)pl[0 * RW] + (float)pl[1 * RW];
         const float im = (float)pl[2 * RW] + (float)pl[3 * RW];
@@ -362,7 +367,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       };
 
       CF4 Pp[2][2], Pm[2][2];  // [kt][LT] cross power
-      static_for<2>([&](auto LT) {
+      static_for<TL>([&](auto LT) {
         // table operands are re-read rather than held across the loops
         asm volatile("" ::: "memory");
         // mean correction V(l) of both images: the stage-2 transform of the
@@ -394,7 +399,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           Vm[X].re = er - orr;
           Vm[X].im = ei - oi;
         });
-        static_for<2>([&](auto kt) {
+        static_for<TL>([&](auto kt) {
           asm volatile("" ::: "memory");
           const int k = 16 * kt + n;
           // stage-2 tile of image X: S(l), S(l + Nh) for the lane's rows
@@ -492,7 +497,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       }
       wave_lds_fence();
       if (n == 0) {
-        static_for<2>([&](auto LT) {
+        static_for<TL>([&](auto LT) {
           static_for<4>([&](auto t) {
             const int l = 16 * LT + 4 * g + t;
             if (l < Nh) {
@@ -509,7 +514,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
 
       // ---------------- inverse 1: A = Q_q (row k, contraction over l)
       unsigned Uh[2][2][2][4], Ul[2][2][2][4];  // [kt][NT][q][t']
-      static_for<2>([&](auto kt) {
+      static_for<TL>([&](auto kt) {
         asm volatile("" ::: "memory");
         static_for<2>([&](auto q) {
           u32x4 rh, rl, ih, il;
@@ -518,10 +523,15 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
               float vr[2], vi[2];
               static_for<2>([&](auto b) {
                 constexpr int t = 2 * decltype(th)::value + decltype(b)::value;
-                const float pr = Pp[kt][LT].re[t], pi = Pp[kt][LT].im[t];
-                const float mr = Pm[kt][LT].re[t], mi = Pm[kt][LT].im[t];
-                vr[int(b)] = (q == 0) ? pr + mr : pr - mr;
-                vi[int(b)] = (q == 0) ? pi + mi : pi - mi;
+                if constexpr (decltype(LT)::value < TL) {
+                  const float pr = Pp[kt][LT].re[t], pi = Pp[kt][LT].im[t];
+                  const float mr = Pm[kt][LT].re[t], mi = Pm[kt][LT].im[t];
+                  vr[int(b)] = (q == 0) ? pr + mr : pr - mr;
+                  vi[int(b)] = (q == 0) ? pi + mi : pi - mi;
+                } else {
+                  vr[int(b)] = 0.f;  // rows l >= 16 do not exist for boxes up to 32
+                  vi[int(b)] = 0.f;
+                }
               });
               unsigned h0, l0, h1, l1;
               split2(vr[0], vr[1], h0, l0);
@@ -535,7 +545,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           const bf16x8 qrh = u4_to_bf8(rh), qrl = u4_to_bf8(rl);
           const bf16x8 qih = u4_to_bf8(ih), qil = u4_to_bf8(il);
           const bf16x8 nqih = neg8(qih), nqil = neg8(qil);
-          static_for<2>([&](auto NT) {
+          static_for<TL>([&](auto NT) {
             const __bf16* B = TBl + ((q == 0) ? C::kBI0 : C::kBI1) + 16 * NT * RS;
             const bf16x8 brh = ld8(B + 0 * PL), brl = ld8(B + 1 * PL);
             const bf16x8 bih = ld8(B + 2 * PL), bil = ld8(B + 3 * PL);
@@ -555,23 +565,27 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
 
       // ---------------- inverse 2: c over (NT, q) column tiles of jj
       f32x4 cv[2][2][2][2];  // [NT][q][IT][pm]
-      static_for<2>([&](auto NT) {
+      static_for<TL>([&](auto NT) {
         static_for<2>([&](auto q) {
           bf16x8 ah[2], al[2];
           static_for<2>([&](auto s) {
             u32x4 vh, vl;
             vh[0] = Uh[0][NT][q][s];
             vh[1] = Uh[0][NT][q][2 + s];
-            vh[2] = Uh[1][NT][q][s];
-            vh[3] = Uh[1][NT][q][2 + s];
             vl[0] = Ul[0][NT][q][s];
             vl[1] = Ul[0][NT][q][2 + s];
-            vl[2] = Ul[1][NT][q][s];
-            vl[3] = Ul[1][NT][q][2 + s];
+            if constexpr (TL > 1) {
+              vh[2] = Uh[1][NT][q][s];
+              vh[3] = Uh[1][NT][q][2 + s];
+              vl[2] = Ul[1][NT][q][s];
+              vl[3] = Ul[1][NT][q][2 + s];
+            } else {
+              vh[2] = vh[3] = vl[2] = vl[3] = 0u;  // no k >= 16 for boxes up to 32
+            }
             ah[s] = u4_to_bf8(vh);
             al[s] = u4_to_bf8(vl);
           });
-          static_for<2>([&](auto IT) {
+          static_for<TL>([&](auto IT) {
             const __bf16* W = TBl + C::kWT + 16 * IT * RS;
             const f32x4 ce = mfma3(ah[0], al[0], ld8(W + 0 * PL), ld8(W + 1 * PL), z4);
             const f32x4 co = mfma3(ah[1], al[1], ld8(W + 2 * PL), ld8(W + 3 * PL), z4);
@@ -586,9 +600,9 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       // jj = 32 NT + 8 g + 2 t + q; in fftshift order (flct.c:1490-1510) it
       // sits at row sx = (x + Nh) mod N, column sy = (jj + Nh) mod N.
       float lmax = 0.f;
-      static_for<2>([&](auto NT) {
+      static_for<TL>([&](auto NT) {
         static_for<2>([&](auto q) {
-          static_for<2>([&](auto IT) {
+          static_for<TL>([&](auto IT) {
             static_for<2>([&](auto pm) {
               const f32x4 v = cv[NT][q][IT][pm];
               lmax = fmaxf(lmax, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
@@ -605,9 +619,9 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       int nge = 0;
       unsigned loc = 0xffffffffu;
       if (gmaxf > 0.f) {
-        static_for<2>([&](auto NT) {
+        static_for<TL>([&](auto NT) {
           static_for<2>([&](auto q) {
-            static_for<2>([&](auto IT) {
+            static_for<TL>([&](auto IT) {
               static_for<2>([&](auto pm) {
                 static_for<4>([&](auto t) {
                   const unsigned long long m = __ballot(fabsf(cv[NT][q][IT][pm][int(t)]) >= thr);
@@ -639,7 +653,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         // gather the 3x3 neighbourhood into LDS
         int dxv[2][2];
         bool xin[2][2];
-        static_for<2>([&](auto IT) {
+        static_for<TL>([&](auto IT) {
           static_for<2>([&](auto pm) {
             const int ii = 16 * IT + n;
             const int sx = (pm == 0) ? ii + Nh : ii;
@@ -647,14 +661,14 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
             xin[IT][pm] = (ii < Nh) && (sx - ixmax >= -1) && (sx - ixmax <= 1);
           });
         });
-        static_for<2>([&](auto NT) {
+        static_for<TL>([&](auto NT) {
           static_for<2>([&](auto q) {
             static_for<4>([&](auto t) {
               const int jj = 32 * NT + 8 * g + 2 * t + q;
               const int sy = (jj < Nh) ? jj + Nh : jj - Nh;
               const int dy = sy - iymax;
               if ((jj < N) && (dy >= -1) && (dy <= 1)) {
-                static_for<2>([&](auto IT) {
+                static_for<TL>([&](auto IT) {
                   static_for<2>([&](auto pm) {
                     if (xin[IT][pm])
                       nb[(dxv[IT][pm] + 1) * 3 + (dy + 1)] = (double)fabsf(cv[NT][q][IT][pm][int(t)]);

@@ -93,6 +93,12 @@ SEG_SHAPES = [
     (150, 131, 10.0, {"kr": 0.4}),                # N = 40 with the filter
     (180, 210, 15.0, {"thresh": 0.3}),            # relative threshold
     (170, 190, 11.0, {"skip": 4, "poff": 1, "qoff": -2, "interp": True}),  # N = 44 lattice
+    # boxes up to 32 use the single-tile variant
+    (60, 77, 2.0, {}),                            # N = 8
+    (70, 81, 3.0, {"kr": 0.6}),                   # N = 12 with the filter
+    (100, 120, 4.0, {}),                          # N = 16
+    (110, 97, 7.0, {"thresh": 0.25}),             # N = 28, relative threshold
+    (140, 150, 8.0, {}),                          # N = 32
 ]
 
 
