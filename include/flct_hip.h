@@ -137,6 +137,21 @@ int flct_hip_last_refined(flct_hip_ctx *ctx, long long *refined);
  * counterpart in the reference. */
 int flct_hip_last_kernel_ms(flct_hip_ctx *ctx, double *main_ms, double *refine_ms);
 
+/* Kernel path of the last track call: "fp64-fft", "fp32-fft",
+ * "bf16-segment" or "bf16-pixel".  A precision form may run on any path
+ * whose error is within the form's contract; FP64 serves every form.  No
+ * counterpart in the reference. */
+const char *flct_hip_last_path(flct_hip_ctx *ctx);
+
+/* Autotuner: runs every kernel path that meets the precision form of opts
+ * on the given device inputs, times the whole call with HIP events, and
+ * makes the fastest the context's path for that form and box size (until
+ * the context is destroyed or tuned again).  Untuned, the split-BF16 form
+ * runs the row-segment kernel for boxes up to 64 and the FP64 kernels above
+ * (measured faster there).  No counterpart in the reference. */
+int flct_hip_autotune(flct_hip_ctx *ctx, const double *d_f1, const double *d_f2, int nx, int ny,
+                      const flct_hip_opts *opts);
+
 /*
  * flct_hip_warp replaces warp_frac2d (warp.c:883-1011): out(i,j) is the
  * image's Fourier series evaluated at (i - shiftx, j - shifty), where with

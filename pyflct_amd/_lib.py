@@ -44,6 +44,8 @@ EXPORTS = (
     "flct_hip_last_stats",
     "flct_hip_last_refined",
     "flct_hip_last_kernel_ms",
+    "flct_hip_last_path",
+    "flct_hip_autotune",
     "flct_hip_warp",
     "flct_hip_warp_device",
     "flct_hip_shift",
@@ -122,6 +124,9 @@ def lib():
         L.flct_hip_last_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong),
                                           ctypes.POINTER(d), ctypes.POINTER(d)]
         L.flct_hip_last_kernel_ms.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.flct_hip_last_path.argtypes = [vp]
+        L.flct_hip_last_path.restype = ctypes.c_char_p
+        L.flct_hip_autotune.argtypes = [vp, vp, vp, i, i, ctypes.POINTER(Opts)]
         L.flct_hip_warp.argtypes = [vp, _dp, _dp, _dp, _dp, i, i, i, i]
         L.flct_hip_warp_device.argtypes = [vp, vp, vp, vp, vp, i, i, i]
         L.flct_hip_shift.argtypes = [vp, _dp, d, d, _dp, i, i, i, i]
@@ -130,7 +135,7 @@ def lib():
         L.flct_hip_upload.argtypes = [vp, vp, vp, ctypes.c_longlong]
         L.flct_hip_download.argtypes = [vp, vp, vp, ctypes.c_longlong]
         for name in EXPORTS:
-            if name not in ("flct_hip_opts_default", "flct_hip_strerror"):
+            if name not in ("flct_hip_opts_default", "flct_hip_strerror", "flct_hip_last_path"):
                 getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib

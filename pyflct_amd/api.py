@@ -208,6 +208,18 @@ class Context:
             raise FlctHipError(rc, f"flct_hip_track_batch (pairs {bad})")
         return [{"tracked": arr[k].tracked, "refined": arr[k].refined} for k in range(len(pairs))]
 
+    def autotune(self, d_f1, d_f2, nx, ny, opts):
+        """Time every kernel path that meets opts' precision form on these
+        device inputs and keep the fastest for that form and box size."""
+        check(_lib.lib().flct_hip_autotune(self._h, d_f1, d_f2, int(nx), int(ny),
+                                           ctypes.byref(opts)), "flct_hip_autotune")
+        return self.last_path()
+
+    def last_path(self):
+        """Kernel path of the last track call ("fp64-fft", "fp32-fft",
+        "bf16-segment", "bf16-pixel")."""
+        return _lib.lib().flct_hip_last_path(self._h).decode()
+
     def last_stats(self):
         n = ctypes.c_longlong()
         k = ctypes.c_double()

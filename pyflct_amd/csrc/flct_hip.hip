@@ -6,6 +6,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <utility>
 #include <vector>
 
 #include "../../include/flct_hip.h"
@@ -42,6 +44,11 @@ struct flct_hip_ctx {
   int seg_n = 0;          // box size of the split tables in segtab
   double seg_sigma = 0.;  // window width of the split tables
   int bf16_kernel = 0;    // FLCT_HIP_BF16_KERNEL: 0 automatic, 1 per-pixel kernels only
+  // kernel path by (precision, box size), chosen by flct_hip_autotune
+  std::map<std::pair<int, int>, int> tuned;
+  int tune_gen = 0;       // bumped by every tuning (part of the batch graph key)
+  int force_path = -1;    // path under test while tuning
+  int last_path = 0;
   long long last_tracked = 0;
   long long refined = 0;  // pixels re-evaluated in FP64 by the last split-BF16 call
   double last_kernel_ms = 0.;
@@ -214,6 +221,47 @@ extern "C" int flct_hip_check_opts(int nx, int ny, const flct_hip_opts* o, int* 
   return FLCT_HIP_OK;
 }
 
+// Kernel paths of the tracker.  A precision form may run on any path whose
+// error is within the form's contract: FP64 serves every form.
+enum Path { PATH_FP64 = 0, PATH_FP32 = 1, PATH_BF16_SEG = 2, PATH_BF16_PIXEL = 3 };
+
+static const char* path_name(int p) {
+  switch (p) {
+    case PATH_FP64: return "fp64-fft";
+    case PATH_FP32: return "fp32-fft";
+    case PATH_BF16_SEG: return "bf16-segment";
+    case PATH_BF16_PIXEL: return "bf16-pixel";
+    default: return "none";
+  }
+}
+
+// candidate paths of a form and box size (the first is the untuned default
+// unless default_path says otherwise)
+static std::vector<int> path_candidates(int precision, int N, int nx, int ny, int bias) {
+  if (precision == FLCT_HIP_FP32) return {PATH_FP32, PATH_FP64};
+  if (precision == FLCT_HIP_SPLIT_BF16) {
+    std::vector<int> v;
+    if (seg_supported(N, nx, ny, bias)) v.push_back(PATH_BF16_SEG);
+    v.push_back(PATH_BF16_PIXEL);
+    v.push_back(PATH_FP64);
+    return v;
+  }
+  return {PATH_FP64};
+}
+
+// Untuned choice: the split-BF16 request on boxes above 64 runs the FP64
+// kernels, which are about 2x faster than the dense-DFT BF16 kernel there
+// (2048^2, boxes 68..84) and exact.
+static int default_path(int precision, int N, int nx, int ny, int bias, int bf16_kernel) {
+  if (precision == FLCT_HIP_FP32) return PATH_FP32;
+  if (precision == FLCT_HIP_SPLIT_BF16) {
+    if (bf16_kernel) return PATH_BF16_PIXEL;
+    if (N > 64) return PATH_FP64;
+    return seg_supported(N, nx, ny, bias) ? PATH_BF16_SEG : PATH_BF16_PIXEL;
+  }
+  return PATH_FP64;
+}
+
 namespace {
 
 // Host plan of one track call: the validated options, the paths taken and
@@ -226,11 +274,13 @@ struct Plan {
   bool relative = false;  // -t below 1 without 'a': times the image maxima (flct.c:594-629)
   bool sparse = false;    // threshold or skip: a tracked-pixel list
   bool seg = false;       // split-BF16 row-segment kernel for interior columns
+  int path = PATH_FP64;   // kernel path
   int ixc = 0, iyc = 0;   // skip-fill lattice size
   std::vector<double> gw, filt;
 };
 
-int make_plan(int nx, int ny, const flct_hip_opts* o, int bf16_kernel, Plan* pl) {
+int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Plan* pl,
+              int force_path = -1) {
   if (!o || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
   if (o->deltat == 0.) return FLCT_HIP_E_ARG;
   int rc = resolve(o, nx, ny, &pl->r);
@@ -276,7 +326,15 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, int bf16_kernel, Plan* pl)
   }
   pl->relative = (o->thresh > 0.) && (o->thresh < 1.) && (!o->thresh_abs);
   pl->sparse = (o->thresh > 0.) || pl->r.skipon;
-  pl->seg = (o->precision == FLCT_HIP_SPLIT_BF16) && !bf16_kernel && seg_supported(N, nx, ny, o->bias);
+  if (force_path >= 0) {
+    pl->path = force_path;
+  } else {
+    const auto it = c->tuned.find({o->precision, N});
+    pl->path = (it != c->tuned.end()) ? it->second
+                                      : default_path(o->precision, N, nx, ny, o->bias, c->bf16_kernel);
+  }
+  if (pl->path == PATH_BF16_SEG && !seg_supported(N, nx, ny, o->bias)) pl->path = PATH_BF16_PIXEL;
+  pl->seg = (pl->path == PATH_BF16_SEG);
   if (pl->r.skipon && o->interp) {
     int ixc = 0, iyc = 0;
     for (int i = 0; i < nx; i++)
@@ -316,7 +374,7 @@ void scratch_sizes(const Plan& pl, size_t sz[7]) {
   sz[0] = pl.sparse ? npix * sizeof(int) : 0;
   sz[1] = sz[2] = (size_t)pl.nx * nsx * 8;
   sz[3] = sz[4] = (size_t)nrx * nsx * 8;
-  sz[5] = (pl.o.precision == FLCT_HIP_SPLIT_BF16) ? npix * sizeof(int) : 0;
+  sz[5] = (pl.path == PATH_BF16_SEG || pl.path == PATH_BF16_PIXEL) ? npix * sizeof(int) : 0;
   sz[6] = pl.seg ? (pl.sparse ? npix : (long long)pl.nx * (N - 1)) * sizeof(int) : 0;
 }
 
@@ -332,7 +390,7 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
   const flct_hip_opts* o = &pl.o;
   const int nx = pl.nx, ny = pl.ny, N = pl.N, M = pl.M;
   const long long npix = (long long)nx * ny;
-  const bool bf16 = (o->precision == FLCT_HIP_SPLIT_BF16);
+  const bool bf16 = (pl.path == PATH_BF16_SEG || pl.path == PATH_BF16_PIXEL);
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, counts, 4);
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned int*>(absmax), 4);
   // image maxima: relative threshold (flct.c:594-629) and the split-BF16 scaling
@@ -423,7 +481,7 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
     tr.flag_list = nullptr;
     tr.flag_count = nullptr;
     rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, tr) : dispatch_track_f64_hi(M, lc, tr);
-  } else if (o->precision == FLCT_HIP_FP32) {
+  } else if (pl.path == PATH_FP32) {
     rc = (M <= 12) ? dispatch_track_f32_lo(M, lc, ta) : dispatch_track_f32_hi(M, lc, ta);
   } else {
     rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, ta) : dispatch_track_f64_hi(M, lc, ta);
@@ -482,7 +540,7 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
                                      double* vm) {
   if (!c || !f1 || !f2 || !o || !vx || !vy || !vm || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
   Plan pl;
-  int rc = make_plan(nx, ny, o, c->bf16_kernel, &pl);
+  int rc = make_plan(nx, ny, o, c, &pl, c->force_path);
   if (rc) return rc;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
@@ -522,7 +580,8 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
   unsigned int hc[4];
   HIPCHK(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  const bool bf16 = (o->precision == FLCT_HIP_SPLIT_BF16);
+  const bool bf16 = (pl.path == PATH_BF16_SEG || pl.path == PATH_BF16_PIXEL);
+  c->last_path = pl.path;
   c->last_tracked = pl.sparse ? (long long)hc[0] : (long long)nx * ny;
   c->refined = bf16 ? (long long)hc[1] : 0;
   c->last_kernel_ms = ev_ms(c->ev[1], c->ev[2]);
@@ -558,6 +617,43 @@ extern "C" int flct_hip_track(flct_hip_ctx* c, const double* f1, const double* f
   return FLCT_HIP_OK;
 }
 
+extern "C" const char* flct_hip_last_path(flct_hip_ctx* c) {
+  return c ? path_name(c->last_path) : "none";
+}
+
+extern "C" int flct_hip_autotune(flct_hip_ctx* c, const double* f1, const double* f2, int nx,
+                                 int ny, const flct_hip_opts* o) {
+  if (!c || !f1 || !f2 || !o || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
+  Plan pl;
+  int rc = make_plan(nx, ny, o, c, &pl);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t bytes = (size_t)nx * ny * 8;
+  if ((rc = ensure(c->vx, bytes)) || (rc = ensure(c->vy, bytes)) || (rc = ensure(c->vm, bytes)))
+    return rc;
+  int best = -1;
+  double best_ms = 0.;
+  for (int p : path_candidates(o->precision, pl.N, nx, ny, o->bias)) {
+    c->force_path = p;
+    double t = 0.;
+    for (int rep = 0; rep < 3 && !rc; ++rep) {  // first run warms up
+      rc = flct_hip_track_device(c, f1, f2, nx, ny, o, (double*)c->vx.p, (double*)c->vy.p,
+                                 (double*)c->vm.p);
+      if (rep > 0) t += c->last_total_ms;
+    }
+    c->force_path = -1;
+    if (rc) return rc;
+    if (best < 0 || t < best_ms) {
+      best = p;
+      best_ms = t;
+    }
+  }
+  c->tuned[{o->precision, pl.N}] = best;
+  c->tune_gen++;
+  c->last_path = best;
+  return FLCT_HIP_OK;
+}
+
 // ------------------------------------------------------------------ batch
 
 static void drop_graph(flct_hip_ctx* c) {
@@ -586,7 +682,7 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
   for (int k = 0; k < npairs; ++k) {
     flct_hip_pair& q = pairs[k];
     int rc = (!q.f1 || !q.f2 || !q.vx || !q.vy || !q.vm) ? FLCT_HIP_E_ARG
-                                                          : make_plan(q.nx, q.ny, &q.opts, c->bf16_kernel, &pl[k]);
+                                                          : make_plan(q.nx, q.ny, &q.opts, c, &pl[k]);
     q.status = rc;
     q.tracked = 0;
     q.refined = 0;
@@ -633,7 +729,8 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
     return rc;
   HIPCHK(hipMemcpyAsync(c->barena.p, arena.data(), arena.size(), hipMemcpyHostToDevice, s));
   // the graph is reused while the pairs are the same
-  std::vector<unsigned char> key((size_t)npairs * sizeof(flct_hip_pair));
+  std::vector<unsigned char> key((size_t)npairs * sizeof(flct_hip_pair) + sizeof(int));
+  std::memcpy(key.data() + (size_t)npairs * sizeof(flct_hip_pair), &c->tune_gen, sizeof(int));
   for (int k = 0; k < npairs; ++k) {
     flct_hip_pair q = pairs[k];
     q.tracked = 0;
@@ -688,7 +785,8 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
   for (int k = 0; k < npairs; ++k) {
     const unsigned int* ck = hc.data() + (size_t)k * 8;
     pairs[k].tracked = pl[k].sparse ? (long long)ck[0] : (long long)pl[k].nx * pl[k].ny;
-    pairs[k].refined = (pl[k].o.precision == FLCT_HIP_SPLIT_BF16) ? (long long)ck[1] : 0;
+    pairs[k].refined =
+        (pl[k].path == PATH_BF16_SEG || pl[k].path == PATH_BF16_PIXEL) ? (long long)ck[1] : 0;
     tracked += pairs[k].tracked;
     refined += pairs[k].refined;
   }

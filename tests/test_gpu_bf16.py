@@ -142,3 +142,38 @@ def test_bf16_segment_matches_pixel_kernel(monkeypatch):
     assert np.array_equal(g[2], p[2])
     rms, mx = _errs(g, p)
     assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
+
+
+def test_paths_and_autotune():
+    """Untuned paths; the autotuner picks among paths that meet the form's
+    contract and later calls of that form and box size use its choice."""
+    from pyflct_amd import api
+    c = api.Context(0)
+    try:
+        f1, f2, _ = synth.cfg4(n=384)
+        ref = c.track(f1, f2, 1.0, 1.0, 15.0)
+        assert c.last_path() == "fp64-fft"
+        c.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
+        assert c.last_path() == "bf16-segment"
+        c.track(f1, f2, 1.0, 1.0, 20.0, precision="split_bf16")  # box 84
+        assert c.last_path() == "fp64-fft"
+        d = [c.malloc(f1.nbytes) for _ in range(2)]
+        try:
+            c.upload(d[0], f1)
+            c.upload(d[1], f2)
+            for form, sigma, allowed in (("split_bf16", 15.0, {"bf16-segment", "bf16-pixel", "fp64-fft"}),
+                                         ("fp32", 9.0, {"fp32-fft", "fp64-fft"}),
+                                         ("fp64", 12.0, {"fp64-fft"})):
+                best = c.autotune(d[0], d[1], 384, 384, c.make_opts(1.0, 1.0, sigma, precision=form))
+                assert best in allowed, (form, best)
+                got = c.track(f1, f2, 1.0, 1.0, sigma, precision=form)
+                assert c.last_path() == best
+                r = ref if sigma == 15.0 else c.track(f1, f2, 1.0, 1.0, sigma)
+                assert np.array_equal(got[2], r[2])
+                rms, _ = _errs(r, got)
+                assert rms <= (RMS_TOL if form == "split_bf16" else 2e-3), (form, best, rms)
+        finally:
+            for p_ in d:
+                c.free(p_)
+    finally:
+        c.close()
