@@ -44,7 +44,8 @@ typedef struct flct_hip_ctx flct_hip_ctx;
 typedef struct flct_hip_opts {
   double deltat;     /* time between images */
   double deltas;     /* length unit of one pixel */
-  double sigma;      /* Gaussian window width in pixels; must be > 0 */
+  double sigma;      /* Gaussian window width in pixels; 0 = one whole-image
+                        correlation (flct.c:361-365), < 0 is an illegal box */
   double thresh;     /* -t value (0: off) */
   int thresh_abs;    /* -t value carried an 'a' suffix: absolute units */
   int skip;          /* -s N (0: off) */
@@ -81,8 +82,10 @@ int flct_hip_destroy(flct_hip_ctx *ctx);
  * cross_cor (flct.c:1325-1649, hires off), the skip fill by interpcc2d
  * (flct.c:958-1120, 2226-2463) and the final xmiss reset.
  * f1, f2: host arrays (nx*ny doubles, C-space).  vx, vy, vm: caller-allocated
- * host arrays of nx*ny doubles.  sigma = 0 (whole-image shift) returns
- * FLCT_HIP_E_UNSUPPORTED.
+ * host arrays of nx*ny doubles.  sigma = 0 correlates the two whole images
+ * once with window 1 and the whole-image means, ignoring the threshold and
+ * skip (flct.c:567-586, 666-675, 802-808, 920-947): the single velocity goes
+ * to vx[0], vy[0], vm[0] (= 1) and the other elements are not written.
  */
 int flct_hip_track(flct_hip_ctx *ctx, const double *f1, const double *f2, int nx, int ny,
                    const flct_hip_opts *opts, double *vx, double *vy, double *vm);
@@ -115,7 +118,8 @@ typedef struct flct_hip_pair {
  * brings the same pairs (arrays, sizes, options).  Every pair is validated
  * first: if any is invalid, nothing runs, each pair's status is set and the
  * first error is returned.  flct_hip_last_stats reports the sum of the
- * tracked pixels and the device time of the graph.
+ * tracked pixels and the device time of the graph.  A pair with sigma = 0
+ * is rejected with FLCT_HIP_E_UNSUPPORTED (call flct_hip_track_device).
  */
 int flct_hip_track_batch(flct_hip_ctx *ctx, flct_hip_pair *pairs, int npairs);
 
@@ -148,7 +152,8 @@ const char *flct_hip_last_path(flct_hip_ctx *ctx);
  * makes the fastest the context's path for that form and box size (until
  * the context is destroyed or tuned again).  Untuned, the split-BF16 form
  * runs the row-segment kernel for boxes up to 64 and the FP64 kernels above
- * (measured faster there).  No counterpart in the reference. */
+ * (measured faster there).  sigma = 0 has one path: nothing to tune.  No
+ * counterpart in the reference. */
 int flct_hip_autotune(flct_hip_ctx *ctx, const double *d_f1, const double *d_f2, int nx, int ny,
                       const flct_hip_opts *opts);
 

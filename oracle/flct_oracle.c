@@ -650,7 +650,6 @@ static int check_opts(const oracle_opts_t *o, int nx, int ny, int *poff, int *qo
                       int *skipon) {
   int p = o->poff, q = o->qoff;
   *skipon = 0;
-  if (!(o->sigma > 0.)) return OR_E_UNSUPPORTED;
   if (o->skip != 0 || p != 0 || q != 0) {
     if (o->skip <= 0) return OR_E_SKIP_NONPOS;
     if ((abs(p) >= o->skip) || (abs(q) >= o->skip)) return OR_E_PQ_RANGE;
@@ -691,8 +690,38 @@ int oracle_resolve_thresh(const double *f1, const double *f2, int n, double thre
   return 0;
 }
 
+/* sigma = 0 (flct.c:361-365, 567-586, 666-675, 802-808, 920-947): one
+ * cross_cor of the whole images with window 1 and the whole-image means
+ * (summed row-major, flct.c:815-827); threshold and skip are ignored; one
+ * velocity, written to vx[0], vy[0], vm[0]. */
+static int flct_whole(const double *f1, const double *f2, int nx, int ny,
+                      const oracle_opts_t *o, double *vx, double *vy, double *vm) {
+  double f1bar = 0., f2bar = 0., shiftx, shifty, deltinv = 1. / o->deltat;
+  double *g1 = (double *)malloc(sizeof(double) * (size_t)nx * ny);
+  double *g2 = (double *)malloc(sizeof(double) * (size_t)nx * ny);
+  int ii, jj;
+  for (ii = 0; ii < nx; ii++)
+    for (jj = 0; jj < ny; jj++) {
+      f1bar = f1bar + f1[ii * ny + jj];
+      f2bar = f2bar + f2[ii * ny + jj];
+    }
+  f1bar = f1bar / ((double)nx * ny);
+  f2bar = f2bar / ((double)nx * ny);
+  for (ii = 0; ii < nx * ny; ii++) {
+    g1[ii] = (double)1. * (f1[ii] - f1bar);
+    g2[ii] = (double)1. * (f2[ii] - f2bar);
+  }
+  oracle_cross_cor(g1, g2, nx, ny, o->filter, o->kr, &shiftx, &shifty, NULL);
+  vx[0] = shifty * deltinv * o->deltas;
+  vy[0] = shiftx * deltinv * o->deltas;
+  vm[0] = 1.;
+  free(g1);
+  free(g2);
+  return OR_OK;
+}
+
 /* The whole per-pixel loop plus skip fill and final reset
- * (flct.c:567-1133) for sigma > 0.  Outputs are C-space nx x ny. */
+ * (flct.c:567-1133).  Outputs are C-space nx x ny (1 x 1 for sigma = 0). */
 int oracle_flct(const double *f1, const double *f2, int nx, int ny, double deltat,
                 double deltas, double sigma, double thresh_in, int thresh_abs,
                 int skip, int poff_in, int qoff_in, int interp, int filter,
@@ -707,6 +736,7 @@ int oracle_flct(const double *f1, const double *f2, int nx, int ny, double delta
   o.filter = filter; o.kr = kr; o.bias = bias;
   rc = check_opts(&o, nx, ny, &poff, &qoff, &skipon);
   if (rc) return rc;
+  if (sigma == 0.) return flct_whole(f1, f2, nx, ny, &o, vx, vy, vm);
   rc = track_ctx_init(&t, nx, ny, &o);
   if (rc) return rc;
   thresh = resolve_thresh(f1, f2, nx * ny, &o);
@@ -803,6 +833,7 @@ int oracle_flct_pixels(const double *f1, const double *f2, int nx, int ny,
   o.filter = filter; o.kr = kr; o.bias = bias;
   rc = check_opts(&o, nx, ny, &poff, &qoff, &skipon);
   if (rc) return rc;
+  if (sigma == 0.) return OR_E_UNSUPPORTED;  /* one whole-image velocity: oracle_flct */
   rc = track_ctx_init(&t, nx, ny, &o);
   if (rc) return rc;
   thresh = resolve_thresh(f1, f2, nx * ny, &o);

@@ -139,12 +139,17 @@ class Context:
 
     # ------------------------------------------------------------ tracking
     def track(self, f1, f2, deltat, deltas, sigma, **kw):
-        """Track on host arrays; returns (vx, vy, vm) float64 arrays."""
+        """Track on host arrays; returns (vx, vy, vm) float64 arrays.
+
+        With ``sigma = 0`` the two whole images are correlated once and the
+        arrays are 1 x 1 (flct.c:581-586).
+        """
         f1, f2 = f64c(f1), f64c(f2)
         if f1.ndim != 2 or f1.shape != f2.shape:
             raise ValueError("f1 and f2 must be 2-D arrays of the same shape")
         nx, ny = f1.shape
-        vx, vy, vm = (np.empty((nx, ny)) for _ in range(3))
+        shape = (1, 1) if sigma == 0 else (nx, ny)
+        vx, vy, vm = (np.empty(shape) for _ in range(3))
         o = self.make_opts(deltat, deltas, sigma, **kw)
         check(_lib.lib().flct_hip_track(self._h, ptr(f1), ptr(f2), nx, ny, ctypes.byref(o),
                                         ptr(vx), ptr(vy), ptr(vm)), "flct_hip_track")

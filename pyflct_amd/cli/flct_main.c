@@ -8,8 +8,8 @@
  * characters; -t with a trailing 'a' is absolute; -s is parsed right to left
  * (i, then q, then p, then N).  The input file holds two vcimage arrays, the
  * output file vx, vy, vm (flct.c:1138).  Errors print a message and exit(1)
- * like the reference.  -b (bias correction) is an addition; -h and sigma = 0
- * are not available in this build.
+ * like the reference.  -b (bias correction) is an addition; -h (hires)
+ * is not available in this build.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -100,10 +100,6 @@ int main(int argc, char *argv[]) {
     printf("flct: the -h (hires) option is not available in this build\n");
     exit(1);
   }
-  if (!(o.sigma > 0.)) {
-    printf("flct: sigma = 0 (whole-image shift) is not available in this build\n");
-    exit(1);
-  }
   if (!quiet) {
     printf("flct: MI355X build\n");
     printf("flct: infile = %s\nflct: outfile = %s\n", argv[1], argv[2]);
@@ -130,7 +126,7 @@ int main(int argc, char *argv[]) {
     printf("flct: %s\n", flct_hip_strerror(rc));
     exit(1);
   }
-  if (!quiet) printf("flct: nominal sliding box size = %d\n", nbox);
+  if (!quiet && o.sigma != 0.) printf("flct: nominal sliding box size = %d\n", nbox);
   vx = (double *)malloc(sizeof(double) * (size_t)nx * ny);
   vy = (double *)malloc(sizeof(double) * (size_t)nx * ny);
   vm = (double *)malloc(sizeof(double) * (size_t)nx * ny);
@@ -153,6 +149,12 @@ int main(int argc, char *argv[]) {
     double kms = 0., tms = 0.;
     flct_hip_last_stats(ctx, &tracked, &kms, &tms);
     printf("flct: tracked %lld pixels in %.3f ms on the GPU\n", tracked, tms);
+  }
+  if (o.sigma == 0.) {
+    /* one velocity for the whole image, written as 1 x 1 arrays (flct.c:581-586) */
+    if (!quiet) printf("\nflct: vx = %g vy = %g \n", vx[0], vy[0]);
+    nx = 1;
+    ny = 1;
   }
   {
     const double *outs[3] = {vx, vy, vm};

@@ -14,6 +14,7 @@
 #include "aux_kernels.hpp"
 #include "track_kernels.hpp"
 #include "warp_kernels.hpp"
+#include "whole_kernels.hpp"
 #include "track_launch.hpp"
 
 using namespace flct;
@@ -179,12 +180,13 @@ namespace {
 struct Resolved {
   int nsize = 0, N = 0;
   int poff = 0, qoff = 0, skipon = 0;
+  int whole = 0;  // sigma = 0: one correlation of the whole image
 };
 
 // argument rules of the reference CLI (flct.c:459-472, 537-538, 559-563,
 // 567-580)
 int resolve(const flct_hip_opts* o, int nx, int ny, Resolved* r) {
-  if (!(o->sigma > 0.)) return FLCT_HIP_E_UNSUPPORTED;
+  if (o->sigma != o->sigma) return FLCT_HIP_E_ARG;
   int p = o->poff, q = o->qoff;
   r->skipon = 0;
   if (o->skip != 0 || p != 0 || q != 0) {
@@ -195,6 +197,12 @@ int resolve(const flct_hip_opts* o, int nx, int ny, Resolved* r) {
   if (p < 0) p = o->skip - std::abs(p);
   if (q < 0) q = o->skip - std::abs(q);
   if ((o->skip >= nx) || (o->skip >= ny)) return FLCT_HIP_E_SKIP_TOO_BIG;
+  r->poff = p;
+  r->qoff = q;
+  if (o->sigma == 0.) {  // flct.c:361-365, 567-586
+    r->whole = 1;
+    return FLCT_HIP_OK;
+  }
   const double tol = 1e-2;
   int nt = (int)o->sigma * std::sqrt(std::log(1. / tol));
   int ndmin = (nx < ny) ? (((nx / 3) / 2)) * 2 : (((ny / 3) / 2)) * 2;
@@ -202,8 +210,6 @@ int resolve(const flct_hip_opts* o, int nx, int ny, Resolved* r) {
   if (nsize <= 0) return FLCT_HIP_E_BOX;
   r->nsize = nsize;
   r->N = 2 * nsize;
-  r->poff = p;
-  r->qoff = q;
   return FLCT_HIP_OK;
 }
 
@@ -215,7 +221,7 @@ extern "C" int flct_hip_check_opts(int nx, int ny, const flct_hip_opts* o, int* 
   const int rc = resolve(o, nx, ny, &r);
   if (rc) return rc;
   if (nbox) *nbox = r.N;
-  if (r.N % 4 != 0 || r.N / 4 > kTrackMaxM) return FLCT_HIP_E_UNSUPPORTED;
+  if (!r.whole && (r.N % 4 != 0 || r.N / 4 > kTrackMaxM)) return FLCT_HIP_E_UNSUPPORTED;
   if (o->precision < FLCT_HIP_FP64 || o->precision > FLCT_HIP_SPLIT_BF16) return FLCT_HIP_E_ARG;
   if (o->deltat == 0.) return FLCT_HIP_E_ARG;
   return FLCT_HIP_OK;
@@ -285,6 +291,7 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Pla
   if (o->deltat == 0.) return FLCT_HIP_E_ARG;
   int rc = resolve(o, nx, ny, &pl->r);
   if (rc) return rc;
+  if (pl->r.whole) return FLCT_HIP_E_UNSUPPORTED;  // sigma = 0 runs track_whole
   if (o->precision != FLCT_HIP_FP64 && o->precision != FLCT_HIP_FP32 &&
       o->precision != FLCT_HIP_SPLIT_BF16)
     return FLCT_HIP_E_UNSUPPORTED;
@@ -535,10 +542,130 @@ static int ensure_scratch(flct_hip_ctx* c, const Plan& pl, Scratch* sc) {
   return FLCT_HIP_OK;
 }
 
+static void root_table(std::vector<double>& t, int n);
+
+// sigma = 0 (flct.c:567-586, 666-675, 802-808, 920-947): one cross_cor of
+// the whole nx x ny images with window 1 and whole-image means; threshold and
+// skip are ignored and the velocity is a single value.  Device inputs; the
+// result (vx, vy, vm) goes to host memory.
+static int track_whole(flct_hip_ctx* c, const double* f1, const double* f2, int nx, int ny,
+                       const flct_hip_opts* o, double out[3]) {
+  const long long n = (long long)nx * ny;
+  if (n > 0x7fffffffLL) return FLCT_HIP_E_UNSUPPORTED;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int rc;
+  std::vector<double> t1, t2;
+  root_table(t1, ny);
+  root_table(t2, nx);
+  if ((rc = ensure(c->w3, n * 16)) || (rc = ensure(c->w4, n * 16)) ||
+      (rc = ensure(c->wtw1, t1.size() * 8)) || (rc = ensure(c->wtw2, t2.size() * 8)) ||
+      (rc = ensure(c->w5, (2 * kSumBlocks + 8) * 8)))
+    return rc;
+  HIPCHK(hipMemcpyAsync(c->wtw1.p, t1.data(), t1.size() * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->wtw2.p, t2.data(), t2.size() * 8, hipMemcpyHostToDevice, s));
+  double* part = (double*)c->w5.p;
+  double* means = part + 2 * kSumBlocks;
+  unsigned long long* red = (unsigned long long*)(means + 2);  // vmax, loc
+  double* z = (double*)c->w3.p;
+  double* t = (double*)c->w4.p;
+  const double* tw1 = (const double*)c->wtw1.p;
+  const double* tw2 = (const double*)c->wtw2.p;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  HIPCHK(hipEventRecord(c->ev[0], s));
+  hipLaunchKernelGGL(k_sum_partial, dim3(kSumBlocks), dim3(256), 0, s, f1, n, part);
+  hipLaunchKernelGGL(k_sum_partial, dim3(kSumBlocks), dim3(256), 0, s, f2, n, part + kSumBlocks);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, s, (const double*)part, kSumBlocks, n, means);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, s, (const double*)(part + kSumBlocks),
+                     kSumBlocks, n, means + 1);
+  hipLaunchKernelGGL(k_pack_centered, dim3(nb), dim3(256), 0, s, f1, f2, (const double*)means, n, z);
+  HIPCHK(hipEventRecord(c->ev[1], s));
+  // forward 2-D transform of z (rows, then columns), e^{-2 pi i}
+  hipLaunchKernelGGL(k_dft<false>, dim3(nb), dim3(256), 0, s, (const double*)z, (long long)ny, 1LL,
+                     ny, ny, nx, t, (long long)ny, 1LL, -1, tw1);
+  hipLaunchKernelGGL(k_dft<false>, dim3(nb), dim3(256), 0, s, (const double*)t, 1LL, (long long)ny,
+                     nx, nx, ny, z, 1LL, (long long)ny, -1, tw2);
+  hipLaunchKernelGGL(k_cross_full, dim3(nb), dim3(256), 0, s, (const double*)z, nx, ny,
+                     o->filter ? 1 : 0, o->kr, t);
+  // inverse (c2r: real part of the backward transform)
+  hipLaunchKernelGGL(k_dft<false>, dim3(nb), dim3(256), 0, s, (const double*)t, (long long)ny, 1LL,
+                     ny, ny, nx, z, (long long)ny, 1LL, +1, tw1);
+  hipLaunchKernelGGL(k_dft<false>, dim3(nb), dim3(256), 0, s, (const double*)z, 1LL, (long long)ny,
+                     nx, nx, ny, t, 1LL, (long long)ny, +1, tw2);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned int*>(red), 4);
+  hipLaunchKernelGGL(k_first_max_init, dim3(1), dim3(64), 0, s, red + 1);
+  double* sh = z;  // shifted |c|, reusing the spectrum buffer
+  hipLaunchKernelGGL(k_abs_shift, dim3(nb), dim3(256), 0, s, (const double*)t, nx, ny, sh, red);
+  hipLaunchKernelGGL(k_first_max, dim3(nb), dim3(256), 0, s, (const double*)sh, n,
+                     (const unsigned long long*)red, red + 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->ev[2], s));
+  unsigned long long hr[2];
+  HIPCHK(hipMemcpyAsync(hr, red, sizeof(hr), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double vmaxd;
+  std::memcpy(&vmaxd, &hr[0], 8);
+  // peak and 3x3 fit (flct.c:1512-1556, 1639-1640)
+  int ixmax, iymax;
+  const bool absccmax = (vmaxd != 0.);
+  if (absccmax) {
+    ixmax = (int)(hr[1] / (unsigned long long)ny);
+    iymax = (int)(hr[1] - (unsigned long long)ixmax * ny);
+  } else {
+    ixmax = nx / 2;
+    iymax = ny / 2;
+  }
+  double shiftsubx = 0., shiftsuby = 0.;
+  if (ixmax > 0 && ixmax < nx - 1 && iymax > 0 && iymax < ny - 1 && absccmax) {
+    double a[3][3];
+    for (int r = 0; r < 3; ++r)
+      HIPCHK(hipMemcpy(a[r], sh + (long long)(ixmax - 1 + r) * ny + (iymax - 1), 3 * 8,
+                       hipMemcpyDeviceToHost));
+    {
+#pragma clang fp contract(off)
+      const double fx = 0.5 * (a[2][1] - a[0][1]);
+      const double fy = 0.5 * (a[1][2] - a[1][0]);
+      const double fxx = (a[2][1] + a[0][1] - 2. * (a[1][1]));
+      const double fyy = (a[1][2] + a[1][0] - 2. * (a[1][1]));
+      const double fxy = 0.25 * (a[2][2] + a[0][0] - a[2][0] - a[0][2]);
+      shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
+      shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
+    }
+  }
+  const double shiftx = ((double)ixmax + shiftsubx) - (double)(nx / 2);
+  const double shifty = ((double)iymax + shiftsuby) - (double)(ny / 2);
+  const double deltinv = 1. / o->deltat;
+  out[0] = shifty * deltinv * o->deltas;  // transp = 1 (flct.c:900-905)
+  out[1] = shiftx * deltinv * o->deltas;
+  out[2] = 1.;
+  c->last_tracked = 1;
+  c->refined = 0;
+  c->last_path = PATH_FP64;
+  c->last_kernel_ms = c->last_main_ms = ev_ms(c->ev[1], c->ev[2]);
+  c->last_refine_ms = 0.;
+  c->last_total_ms = ev_ms(c->ev[0], c->ev[2]);
+  return FLCT_HIP_OK;
+}
+
 extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const double* f2, int nx,
                                      int ny, const flct_hip_opts* o, double* vx, double* vy,
                                      double* vm) {
   if (!c || !f1 || !f2 || !o || !vx || !vy || !vm || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
+  {
+    Resolved r0;
+    const int rc0 = resolve(o, nx, ny, &r0);
+    if (rc0) return rc0;
+    if (r0.whole) {
+      if (o->deltat == 0.) return FLCT_HIP_E_ARG;
+      double out[3];
+      const int rc = track_whole(c, f1, f2, nx, ny, o, out);
+      if (rc) return rc;
+      HIPCHK(hipMemcpy(vx, &out[0], 8, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(vy, &out[1], 8, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(vm, &out[2], 8, hipMemcpyHostToDevice));
+      return FLCT_HIP_OK;
+    }
+  }
   Plan pl;
   int rc = make_plan(nx, ny, o, c, &pl, c->force_path);
   if (rc) return rc;
@@ -602,6 +729,21 @@ extern "C" int flct_hip_track(flct_hip_ctx* c, const double* f1, const double* f
   HIPCHK(hipSetDevice(c->device));
   const size_t bytes = (size_t)nx * ny * sizeof(double);
   int rc;
+  if (o->sigma == 0.) {  // one velocity for the whole image: vx[0], vy[0], vm[0]
+    Resolved r0;
+    if ((rc = resolve(o, nx, ny, &r0))) return rc;
+    if (o->deltat == 0.) return FLCT_HIP_E_ARG;
+    if ((rc = ensure(c->f1, bytes)) || (rc = ensure(c->f2, bytes))) return rc;
+    HIPCHK(hipMemcpyAsync(c->f1.p, f1, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->f2.p, f2, bytes, hipMemcpyHostToDevice, c->stream));
+    double out[3];
+    if ((rc = track_whole(c, (const double*)c->f1.p, (const double*)c->f2.p, nx, ny, o, out)))
+      return rc;
+    vx[0] = out[0];
+    vy[0] = out[1];
+    vm[0] = out[2];
+    return FLCT_HIP_OK;
+  }
   if ((rc = ensure(c->f1, bytes)) || (rc = ensure(c->f2, bytes)) || (rc = ensure(c->vx, bytes)) ||
       (rc = ensure(c->vy, bytes)) || (rc = ensure(c->vm, bytes)))
     return rc;
@@ -624,6 +766,12 @@ extern "C" const char* flct_hip_last_path(flct_hip_ctx* c) {
 extern "C" int flct_hip_autotune(flct_hip_ctx* c, const double* f1, const double* f2, int nx,
                                  int ny, const flct_hip_opts* o) {
   if (!c || !f1 || !f2 || !o || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
+  {
+    Resolved r0;
+    const int rc0 = resolve(o, nx, ny, &r0);
+    if (rc0) return rc0;
+    if (r0.whole) return FLCT_HIP_OK;  // one path for sigma = 0
+  }
   Plan pl;
   int rc = make_plan(nx, ny, o, c, &pl);
   if (rc) return rc;

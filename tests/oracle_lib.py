@@ -82,10 +82,12 @@ def _f64(a):
 
 def flct(f1, f2, deltat, deltas, sigma, thresh=0.0, thresh_abs=False, skip=0,
          poff=0, qoff=0, interp=False, kr=None, bias=False):
-    """Whole flct run on C-space arrays; returns (vx, vy, vm)."""
+    """Whole flct run on C-space arrays; returns (vx, vy, vm), 1 x 1 for
+    sigma = 0."""
     f1, f2 = _f64(f1), _f64(f2)
     nx, ny = f1.shape
-    vx, vy, vm = (np.empty((nx, ny)) for _ in range(3))
+    shape = (1, 1) if sigma == 0 else (nx, ny)
+    vx, vy, vm = (np.empty(shape) for _ in range(3))
     rc = lib().oracle_flct(_ptr(f1), _ptr(f2), nx, ny, deltat, deltas, sigma, thresh,
                            int(thresh_abs), skip, poff, qoff, int(interp),
                            int(kr is not None), float(kr or 0.0), int(bias),

@@ -90,6 +90,29 @@ def test_flct_cli_options_match_api(tmp_path, ctx):
 
 
 @pytest.mark.gpu
+def test_flct_cli_sigma0_writes_one_velocity(tmp_path, ctx):
+    """flct in.dat out.dat 1 1 0 -k 0.5 (flct.1:212-219): the net shift of
+    the whole image, written as 1 x 1 arrays."""
+    from pyflct_amd import vcimage
+    rng = np.random.default_rng(4)
+    from scipy.ndimage import gaussian_filter
+    f1 = gaussian_filter(rng.standard_normal((70, 90)), 2.0, mode="wrap")
+    f2 = np.roll(f1, (1, -2), axis=(0, 1))
+    inp = tmp_path / "in.dat"
+    vcimage.write(inp, f1, f2)
+    g1, g2 = vcimage.read2(inp)
+    out = tmp_path / "out.dat"
+    rc, text = run(FLCT, str(inp), str(out), "1", "1", "0", "-k", "0.5")
+    assert rc == 0, text
+    assert "vx = " in text and "box size" not in text
+    r = vcimage.read3(out)
+    g = ctx.track(g1, g2, 1.0, 1.0, 0.0, kr=0.5)
+    for a, b in zip(r, g):
+        assert a.shape == (1, 1) and np.array_equal(a, f32(b))
+    assert abs(r[0][0, 0] + 2) < 1e-5 and abs(r[1][0, 0] - 1) < 1e-5
+
+
+@pytest.mark.gpu
 def test_warp_cli_warp_and_uniform_shift(tmp_path, golden):
     from pyflct_amd import vcimage
     out = tmp_path / "w.dat"

@@ -206,7 +206,8 @@ def test_error_codes(ctx):
     cases = [({"sigma": 4.0, "skip": 50}, L.FLCT_HIP_E_SKIP_TOO_BIG),
              ({"sigma": 4.0, "skip": 4, "poff": 5}, L.FLCT_HIP_E_PQ_RANGE),
              ({"sigma": 4.0, "skip": -1}, L.FLCT_HIP_E_SKIP_NONPOS),
-             ({"sigma": 0.0}, L.FLCT_HIP_E_UNSUPPORTED)]
+             ({"sigma": -3.0}, L.FLCT_HIP_E_BOX),
+             ({"sigma": 0.0, "skip": 40}, L.FLCT_HIP_E_SKIP_TOO_BIG)]
     for kw, code in cases:
         sig = kw.pop("sigma")
         with pytest.raises(FlctHipError) as e:
@@ -215,3 +216,55 @@ def test_error_codes(ctx):
     with pytest.raises(FlctHipError) as e:
         ctx.track(np.zeros((4, 4)), np.zeros((4, 4)), 1.0, 1.0, 5.0)
     assert e.value.status == L.FLCT_HIP_E_BOX
+
+
+# ------------------------------------------------------------- sigma = 0
+def _whole_pair(nx, ny, seed, shift=(0.35, -0.6)):
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    g = gaussian_filter(rng.standard_normal((nx, ny)), 2.5, mode="wrap") + 3.0
+    kx = np.fft.fftfreq(nx)[:, None]
+    ky = np.fft.fftfreq(ny)[None, :]
+    h = np.fft.ifft2(np.fft.fft2(g) * np.exp(-2j * np.pi * (kx * shift[0] + ky * shift[1]))).real
+    return g, h + 0.02 * rng.standard_normal((nx, ny))
+
+
+@pytest.mark.parametrize("nx,ny,kw", [(64, 64, {}), (101, 201, {}), (127, 96, {"kr": 0.5}),
+                                      (250, 333, {"deltat": 2.0, "deltas": 0.7}),
+                                      (512, 384, {"kr": 0.3})])
+def test_sigma0_whole_image(ctx, nx, ny, kw):
+    """sigma = 0: one correlation of the whole images (flct.c:567-586,
+    802-808, 920-947), any nx, ny: the single velocity against the oracle."""
+    kw = dict(kw)
+    dt, ds = kw.pop("deltat", 1.0), kw.pop("deltas", 1.0)
+    f1, f2 = _whole_pair(nx, ny, seed=nx + ny)
+    g = ctx.track(f1, f2, dt, ds, 0.0, **kw)
+    o = O.flct(f1, f2, dt, ds, 0.0, **kw)
+    assert g[0].shape == (1, 1)
+    assert_parity(g, o, where=f"sigma=0 {nx}x{ny} {kw}")
+    assert abs(g[0][0, 0] * dt / ds + 0.6) < 0.1 and abs(g[1][0, 0] * dt / ds - 0.35) < 0.1
+
+
+def test_sigma0_ignores_threshold_skip_and_takes_device_arrays(ctx):
+    f1, f2 = _whole_pair(90, 110, seed=7, shift=(-1.3, 2.2))
+    ref = ctx.track(f1, f2, 1.0, 1.0, 0.0)
+    for kw in ({"thresh": 0.9}, {"thresh": 1e6, "thresh_abs": True}, {"skip": 6, "qoff": 2}):
+        g = ctx.track(f1, f2, 1.0, 1.0, 0.0, **kw)
+        for a, b in zip(ref, g):
+            assert np.array_equal(a, b), kw
+    # device entry point: the velocity goes to element 0 of each output
+    d = [ctx.malloc(f1.nbytes) for _ in range(2)] + [ctx.malloc(8) for _ in range(3)]
+    try:
+        ctx.upload(d[0], f1)
+        ctx.upload(d[1], f2)
+        ctx.track_device(d[0], d[1], 90, 110, d[2], d[3], d[4], ctx.make_opts(1.0, 1.0, 0.0))
+        got = [ctx.download(np.empty(1), d[k]) for k in (2, 3, 4)]
+        for a, b in zip(ref, got):
+            assert np.array_equal(a.ravel(), b)
+    finally:
+        for p in d:
+            ctx.free(p)
+    # zero images: the maximum-zero branch puts the peak at the centre
+    z = np.zeros((33, 48))
+    assert [float(v[0, 0]) for v in ctx.track(z, z, 1.0, 1.0, 0.0)] == [0.0, 0.0, 1.0]
+    assert_parity(ctx.track(z, z, 1.0, 1.0, 0.0), O.flct(z, z, 1.0, 1.0, 0.0))

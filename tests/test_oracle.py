@@ -231,3 +231,58 @@ def test_skip_lattice_and_fill_mask():
         _, _, vmi = O.flct(a, b, 1.0, 1.0, 4.0, skip=skip, poff=p, qoff=q, interp=True)
         assert np.all(vmi[lattice] == 1)
         assert set(np.unique(vmi[~lattice])) <= {0.0, 0.5}
+
+
+def _fourier_shift(a, dx, dy):
+    """Band-limited circular shift of ``a`` by (dx, dy) pixels along (axis 0,
+    axis 1), with numpy's FFT."""
+    kx = np.fft.fftfreq(a.shape[0])[:, None]
+    ky = np.fft.fftfreq(a.shape[1])[None, :]
+    return np.fft.ifft2(np.fft.fft2(a) * np.exp(-2j * np.pi * (kx * dx + ky * dy))).real
+
+
+@pytest.mark.parametrize("nx,ny", [(48, 64), (51, 37), (101, 60)])
+def test_sigma0_whole_image_integer_shift(nx, ny):
+    """sigma = 0 correlates the two whole images once (flct.c:567-586,
+    802-808): a circular integer shift gives a correlation surface symmetric
+    about the peak, so the 3 x 3 fit adds nothing and the velocity is the
+    shift times deltas / deltat, with x and y swapped (transp = 1)."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(nx * 1000 + ny)
+    g = gaussian_filter(rng.standard_normal((nx, ny)), 2.0, mode="wrap") + 5.0
+    h = np.roll(g, (2, -3), axis=(0, 1)) + 1.25  # an offset is removed by the mean
+    for kw in ({}, {"kr": 0.6}):
+        vx, vy, vm = O.flct(g, h, 2.0, 0.5, 0.0, **kw)
+        assert vx.shape == (1, 1) and vm[0, 0] == 1.0
+        assert abs(vx[0, 0] - (-3) * 0.25) < 1e-9, kw
+        assert abs(vy[0, 0] - 2 * 0.25) < 1e-9, kw
+
+
+def test_sigma0_ignores_threshold_and_skip():
+    """sigma = 0 computes the one velocity whatever the threshold and the
+    skip lattice say (flct.c:594, 920)."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(11)
+    g = gaussian_filter(rng.standard_normal((40, 44)), 1.5, mode="wrap")
+    h = np.roll(g, (-1, 2), axis=(0, 1))
+    ref = O.flct(g, h, 1.0, 1.0, 0.0)
+    for kw in ({"thresh": 0.9}, {"thresh": 100.0, "thresh_abs": True}, {"skip": 5, "poff": 2}):
+        got = O.flct(g, h, 1.0, 1.0, 0.0, **kw)
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b), kw
+
+
+def test_sigma0_subpixel_shift_and_negative_sigma():
+    """A band-limited sub-pixel shift is recovered to the accuracy of the
+    quadratic peak fit; a negative sigma is an illegal box (flct.c:573-579)."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(12)
+    g = gaussian_filter(rng.standard_normal((96, 80)), 3.0, mode="wrap")
+    h = _fourier_shift(g, 0.3, -0.45)
+    vx, vy, _ = O.flct(g, h, 1.0, 1.0, 0.0)
+    assert abs(vx[0, 0] + 0.45) < 0.05 and abs(vy[0, 0] - 0.3) < 0.05
+    h2 = _fourier_shift(g, -0.3, 0.45)
+    vx2, vy2, _ = O.flct(g, h2, 1.0, 1.0, 0.0)
+    assert abs(vx2[0, 0] + vx[0, 0]) < 1e-6 and abs(vy2[0, 0] + vy[0, 0]) < 1e-6
+    with pytest.raises(RuntimeError):
+        O.flct(g, h, 1.0, 1.0, -4.0)
