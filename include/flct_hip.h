@@ -39,7 +39,7 @@ extern "C" {
 typedef struct flct_hip_ctx flct_hip_ctx;
 
 /* Options of one flct run: the reference command line
- * "flct ifile ofile deltat deltas sigma [-t thr[a]] [-k kr] [-s N[pP][qQ][i]] [-b] [-q]"
+ * "flct ifile ofile deltat deltas sigma [-t thr[a]] [-k kr] [-s N[pP][qQ][i]] [-h] [-b] [-q]"
  * (flct.c:304, flct.c:370-487). */
 typedef struct flct_hip_opts {
   double deltat;     /* time between images */
@@ -57,6 +57,9 @@ typedef struct flct_hip_opts {
   int bias;          /* -b: window-offset bias correction (not in the reference) */
   int precision;     /* FLCT_HIP_FP64, FLCT_HIP_FP32, FLCT_HIP_SPLIT_BF16 */
   int quiet;         /* -q: no progress messages */
+  int hires;         /* -h: peak by cubic-convolution interpolation of the
+                        correlation on a 0.1 px grid (flct.c:383, 1357-1368,
+                        1564-1645); runs the FP64 kernels for every precision */
 } flct_hip_opts;
 
 /* Fill *opts with the reference defaults (flct.c:370-375): thresh 0, no
@@ -79,7 +82,7 @@ int flct_hip_destroy(flct_hip_ctx *ctx);
  * flct_hip_track replaces the per-pixel body of flct's main program,
  * flct.c:567-1133: box size, relative threshold, the i,j loop with its
  * threshold and skip rules, the windowed FFT cross-correlation of
- * cross_cor (flct.c:1325-1649, hires off), the skip fill by interpcc2d
+ * cross_cor (flct.c:1325-1649, with or without -h), the skip fill by interpcc2d
  * (flct.c:958-1120, 2226-2463) and the final xmiss reset.
  * f1, f2: host arrays (nx*ny doubles, C-space).  vx, vy, vm: caller-allocated
  * host arrays of nx*ny doubles.  sigma = 0 correlates the two whole images

@@ -21,6 +21,8 @@
  *     hand-computed border cases, box-size and make_freq closed forms;
  *   - invariants (identical images, negated images, swapped images,
  *     integer-shift warp == roll, uniform warp == shift_frac2d).
+ *   - the -h peak (oracle_peak_hires) on sampled quadratics, whose cubic
+ *     convolution is exact, and its all-zero and edge rules;
  *   - the -b window-offset bias correction has no reference counterpart:
  *     parity unpinned (only invariants check it).
  *
@@ -512,6 +514,76 @@ int oracle_interpcc2d(const double *fdata, double xmiss, int nx, int ny,
   return 0;
 }
 
+/* peak of cross_cor with -h (hires = 0): the integer peak as in xcor_peak
+ * (flct.c:1512-1534), no fit there (flct.c:1536), then interpcc2d of the
+ * whole surface with xmiss = 0 on a 21 x 21 grid of 0.1 px about it
+ * (flct.c:1357-1368, 1564-1590), maxloc of that grid (flct.c:1598-1600) and
+ * the 3 x 3 fit there scaled to pixels (flct.c:1605-1631, 1639-1640). */
+static void xcor_peak_hires(const double *a, int nx, int ny, double *shiftx,
+                            double *shifty) {
+  int nxinterp = 21, nyinterp = 21, nfgppergp = 10;
+  int maxind, ixmax, iymax, absccmax = 1, i, j, maxfine, ixx, iyy;
+  double shiftx0, shifty0, rangex, rangey, shiftsubx = 0., shiftsuby = 0.;
+  double xwant[21], ywant[21], peakarea[21 * 21];
+  maxind = maxloc(a, nx * ny);
+  if (a[maxind] == (double)0.) absccmax = 0;
+  if (absccmax == 1) {
+    ixmax = maxind / ny;
+    iymax = maxind % ny;
+  } else {
+    ixmax = nx / 2;
+    iymax = ny / 2;
+  }
+  shiftx0 = ixmax;
+  shifty0 = iymax;
+  rangex = (double)(nxinterp - 1) / nfgppergp;
+  rangey = (double)(nyinterp - 1) / nfgppergp;
+  for (i = 0; i < nxinterp; i++)
+    xwant[i] = ((((double)i) * rangex) / ((double)(nxinterp - 1))) - 0.5 * rangex + shiftx0;
+  for (j = 0; j < nyinterp; j++)
+    ywant[j] = ((((double)j) * rangey) / ((double)(nyinterp - 1))) - 0.5 * rangey + shifty0;
+  oracle_interpcc2d(a, 0., nx, ny, xwant, nxinterp, ywant, nyinterp, peakarea);
+  maxfine = maxloc(peakarea, nxinterp * nyinterp);
+  ixx = maxfine / nyinterp;
+  iyy = maxfine % nyinterp;
+  if ((ixx > 0) && (ixx < (nxinterp - 1)) && (iyy > 0) && (iyy < (nyinterp - 1))) {
+    const double *p = peakarea;
+    double fx, fy, fxx, fyy, fxy;
+    fx = 0.5 * (p[(ixx + 1) * nyinterp + iyy] - p[(ixx - 1) * nyinterp + iyy]);
+    fy = 0.5 * (p[ixx * nyinterp + iyy + 1] - p[ixx * nyinterp + iyy - 1]);
+    fxx = (p[(ixx + 1) * nyinterp + iyy] + p[(ixx - 1) * nyinterp + iyy] -
+           2. * (p[ixx * nyinterp + iyy]));
+    fyy = (p[ixx * nyinterp + iyy + 1] + p[ixx * nyinterp + iyy - 1] -
+           2. * (p[ixx * nyinterp + iyy]));
+    fxy = 0.25 * (p[(ixx + 1) * nyinterp + iyy + 1] + p[(ixx - 1) * nyinterp + iyy - 1] -
+                  p[(ixx + 1) * nyinterp + iyy - 1] - p[(ixx - 1) * nyinterp + iyy + 1]);
+#ifdef PLANT_HIRES_SCALE
+    shiftsubx = ((fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy));
+    shiftsuby = ((fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy));
+#else
+    shiftsubx = ((fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy)) * rangex / ((double)(nxinterp - 1));
+    shiftsuby = ((fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy)) * rangey / ((double)(nyinterp - 1));
+#endif
+  }
+  *shiftx = (xwant[ixx] + shiftsubx) - (double)(nx / 2);
+  *shifty = (ywant[iyy] + shiftsuby) - (double)(ny / 2);
+}
+
+/* exported: the hires peak alone on an nx x ny surface (already shifted) */
+int oracle_peak_hires(const double *a, int nx, int ny, double *shiftx, double *shifty) {
+  xcor_peak_hires(a, nx, ny, shiftx, shifty);
+  return 0;
+}
+
+/* the peak step of cross_cor for the -h setting */
+static void cc_peak(int hires, const double *a, int nx, int ny, double *shiftx,
+                    double *shifty) {
+  if (hires)
+    xcor_peak_hires(a, nx, ny, shiftx, shifty);
+  else
+    xcor_peak(a, nx, ny, shiftx, shifty);
+}
+
 /* ------------------------------------------------------------------ */
 /* per-pixel driver: flct.c:567-1133                                   */
 /* ------------------------------------------------------------------ */
@@ -524,6 +596,7 @@ typedef struct {
   int filter;
   double kr;
   int bias;
+  int hires;
 } oracle_opts_t;
 
 /* status codes shared with include/flct_hip.h */
@@ -587,7 +660,7 @@ static void track_pixel(track_ctx_t *t, const double *f1, const double *f2, int 
       t->g2[ii * jsize + jj] = g * (f2[(ii + imin) * ny + (jj + jmin)] - f2bar);
     }
   xcor_surface(&t->ws, t->g1, t->g2);
-  xcor_peak(t->ws.shifted, isize, jsize, &shiftx, &shifty);
+  cc_peak(o->hires, t->ws.shifted, isize, jsize, &shiftx, &shifty);
   if (o->bias) {
     double cx = 0.5 * shiftx, cy = 0.5 * shifty;
     for (ii = 0; ii < isize; ii++) {
@@ -602,7 +675,7 @@ static void track_pixel(track_ctx_t *t, const double *f1, const double *f2, int 
       }
     }
     xcor_surface(&t->ws, t->g1, t->g2);
-    xcor_peak(t->ws.shifted, isize, jsize, &shiftx, &shifty);
+    cc_peak(o->hires, t->ws.shifted, isize, jsize, &shiftx, &shifty);
   }
   /* transp = 1 (flct.c:901-905) */
 #ifdef PLANT_NO_SWAP
@@ -711,7 +784,13 @@ static int flct_whole(const double *f1, const double *f2, int nx, int ny,
     g1[ii] = (double)1. * (f1[ii] - f1bar);
     g2[ii] = (double)1. * (f2[ii] - f2bar);
   }
-  oracle_cross_cor(g1, g2, nx, ny, o->filter, o->kr, &shiftx, &shifty, NULL);
+  {
+    xcor_ws_t w;
+    ws_init(&w, nx, ny, o->filter, o->kr);
+    xcor_surface(&w, g1, g2);
+    cc_peak(o->hires, w.shifted, nx, ny, &shiftx, &shifty);
+    ws_free(&w);
+  }
   vx[0] = shifty * deltinv * o->deltas;
   vy[0] = shiftx * deltinv * o->deltas;
   vm[0] = 1.;
@@ -725,7 +804,7 @@ static int flct_whole(const double *f1, const double *f2, int nx, int ny,
 int oracle_flct(const double *f1, const double *f2, int nx, int ny, double deltat,
                 double deltas, double sigma, double thresh_in, int thresh_abs,
                 int skip, int poff_in, int qoff_in, int interp, int filter,
-                double kr, int bias, double *vx, double *vy, double *vm) {
+                double kr, int bias, int hires, double *vx, double *vy, double *vm) {
   oracle_opts_t o;
   track_ctx_t t;
   int poff, qoff, skipon, rc, i, j, xoffset, yoffset;
@@ -733,7 +812,7 @@ int oracle_flct(const double *f1, const double *f2, int nx, int ny, double delta
   o.deltat = deltat; o.deltas = deltas; o.sigma = sigma;
   o.thresh = thresh_in; o.thresh_abs = thresh_abs;
   o.skip = skip; o.poff = poff_in; o.qoff = qoff_in; o.interp = interp;
-  o.filter = filter; o.kr = kr; o.bias = bias;
+  o.filter = filter; o.kr = kr; o.bias = bias; o.hires = hires;
   rc = check_opts(&o, nx, ny, &poff, &qoff, &skipon);
   if (rc) return rc;
   if (sigma == 0.) return flct_whole(f1, f2, nx, ny, &o, vx, vy, vm);
@@ -820,7 +899,7 @@ int oracle_flct(const double *f1, const double *f2, int nx, int ny, double delta
 int oracle_flct_pixels(const double *f1, const double *f2, int nx, int ny,
                        double deltat, double deltas, double sigma, double thresh_in,
                        int thresh_abs, int skip, int poff_in, int qoff_in,
-                       int filter, double kr, int bias, const long long *pix,
+                       int filter, double kr, int bias, int hires, const long long *pix,
                        long long npix, double *vx, double *vy, double *vm) {
   oracle_opts_t o;
   track_ctx_t t;
@@ -830,7 +909,7 @@ int oracle_flct_pixels(const double *f1, const double *f2, int nx, int ny,
   o.deltat = deltat; o.deltas = deltas; o.sigma = sigma;
   o.thresh = thresh_in; o.thresh_abs = thresh_abs;
   o.skip = skip; o.poff = poff_in; o.qoff = qoff_in; o.interp = 0;
-  o.filter = filter; o.kr = kr; o.bias = bias;
+  o.filter = filter; o.kr = kr; o.bias = bias; o.hires = hires;
   rc = check_opts(&o, nx, ny, &poff, &qoff, &skipon);
   if (rc) return rc;
   if (sigma == 0.) return OR_E_UNSUPPORTED;  /* one whole-image velocity: oracle_flct */

@@ -79,6 +79,7 @@ class Opts(ctypes.Structure):
         ("bias", ctypes.c_int),
         ("precision", ctypes.c_int),
         ("quiet", ctypes.c_int),
+        ("hires", ctypes.c_int),
     ]
 
 

@@ -115,7 +115,8 @@ class Context:
     # ------------------------------------------------------------ options
     @staticmethod
     def make_opts(deltat, deltas, sigma, thresh=0.0, thresh_abs=False, kr=None, skip=0,
-                  poff=0, qoff=0, interp=False, bias=False, precision="fp64", quiet=True):
+                  poff=0, qoff=0, interp=False, bias=False, precision="fp64", quiet=True,
+                  hires=False):
         o = Opts()
         _lib.lib().flct_hip_opts_default(ctypes.byref(o))
         o.deltat, o.deltas, o.sigma = float(deltat), float(deltas), float(sigma)
@@ -126,6 +127,7 @@ class Context:
         o.bias = int(bool(bias))
         o.precision = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         o.quiet = int(bool(quiet))
+        o.hires = int(bool(hires))
         return o
 
     @staticmethod

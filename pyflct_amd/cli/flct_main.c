@@ -1,15 +1,14 @@
 /*
  * flct -- drop-in command line for the MI355X tracker.
  *
- *   flct ifile ofile deltat deltas sigma [-t thr[a]] [-k kr] [-s N[pP][qQ][i]] [-b] [-q]
+ *   flct ifile ofile deltat deltas sigma [-t thr[a]] [-k kr] [-s N[pP][qQ][i]] [-h] [-b] [-q]
  *
  * Same argument grammar as the reference program (flct.c:298-487): options
  * in any order after the five required arguments, matched on their first two
  * characters; -t with a trailing 'a' is absolute; -s is parsed right to left
  * (i, then q, then p, then N).  The input file holds two vcimage arrays, the
  * output file vx, vy, vm (flct.c:1138).  Errors print a message and exit(1)
- * like the reference.  -b (bias correction) is an addition; -h (hires)
- * is not available in this build.
+ * like the reference.  -b (bias correction) is an addition.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -20,7 +19,7 @@
 
 static void usage(const char *prog) {
   printf("flct (MI355X build): local correlation tracking between two images\n\n");
-  printf("usage: %s ifile ofile deltat deltas sigma [-t thr[a]] [-k kr] [-s N[pP][qQ][i]] [-b] [-q]\n\n",
+  printf("usage: %s ifile ofile deltat deltas sigma [-t thr[a]] [-k kr] [-s N[pP][qQ][i]] [-h] [-b] [-q]\n\n",
          prog);
   printf("  ifile   two images (vcimage format)\n");
   printf("  ofile   vx, vy, vm (vm: 1 computed, 0 not computed, 0.5 interpolated)\n");
@@ -30,6 +29,7 @@ static void usage(const char *prog) {
   printf("          relative to the largest |pixel| unless followed by 'a'\n");
   printf("  -k kr   Gaussian low-pass of the sub-images, kr in units of the Nyquist wavenumber\n");
   printf("  -s NpPqQ  track every N-th pixel (offsets P in x, Q in y); 'i' interpolates the rest\n");
+  printf("  -h      peak by cubic-convolution interpolation on a 0.1 pixel grid\n");
   printf("  -b      window-offset bias correction\n");
   printf("  -q      quiet\n");
 }
@@ -96,10 +96,7 @@ int main(int argc, char *argv[]) {
     }
   }
   o.quiet = quiet;
-  if (hires) {
-    printf("flct: the -h (hires) option is not available in this build\n");
-    exit(1);
-  }
+  o.hires = hires;
   if (!quiet) {
     printf("flct: MI355X build\n");
     printf("flct: infile = %s\nflct: outfile = %s\n", argv[1], argv[2]);
@@ -109,6 +106,7 @@ int main(int argc, char *argv[]) {
     if (o.skip) printf("flct: skip = %d pixels with p=%d, q=%d\n", o.skip, o.poff, o.qoff);
     if (o.interp) printf("flct: skipped pixels interpolated with cubic convolution\n");
     if (o.filter) printf("flct: filter rolloff value for input images is %g\n", o.kr);
+    if (o.hires) printf("flct: hires option turned on\n");
     if (o.bias) printf("flct: bias correction on\n");
   }
   if (vc_read("read2images", argv[1], 2, &nx, &ny, in)) exit(1);

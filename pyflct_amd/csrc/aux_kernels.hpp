@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "interp_cc.hpp"
 #include "track_kernels.hpp"
 
 namespace flct {
@@ -110,14 +111,6 @@ __global__ void k_select(SelectArgs a) {
   }
 }
 
-// Keys cubic convolution weight, a = -0.5 (flct.c:2391-2395)
-__device__ __forceinline__ double keys_w(double t) {
-#pragma clang fp contract(off)
-  return (t >= (double)1.0)
-             ? (((((double)(-0.5)) * t + ((double)2.5)) * t) - (double)4.) * t + (double)2.
-             : (((double)(1.5)) * t - ((double)(2.5))) * t * t + (double)1.;
-}
-
 struct FillArgs {
   int nx, ny;
   int skip, xoffset, yoffset;
@@ -132,70 +125,10 @@ __device__ __forceinline__ double lat(const double* v, const FillArgs& a, int ic
   return v[(long long)(a.xoffset + ic * a.skip) * a.ny + (a.yoffset + jc * a.skip)];
 }
 
-// cdata of interpcc2d (flct.c:2266-2336): lattice padded by one cell with
-// the reference's edge and corner extrapolation, index (ci, cj) in
-// [0, ixcount+1] x [0, iycount+1]
-__device__ double cdata_at(const double* v, const FillArgs& a, int ci, int cj) {
-#pragma clang fp contract(off)
-  const int nxl = a.ixcount, nyl = a.iycount;
-  const bool ie = (ci == 0 || ci == nxl + 1), je = (cj == 0 || cj == nyl + 1);
-  if (!ie && !je) return lat(v, a, ci - 1, cj - 1);
-  if (ie && !je) {
-    const int jc = cj - 1;
-    if (ci == 0)
-      return lat(v, a, 2, jc) - 3. * (lat(v, a, 1, jc)) + 3. * (lat(v, a, 0, jc));
-    return lat(v, a, nxl - 3, jc) - 3. * (lat(v, a, nxl - 2, jc)) + 3. * (lat(v, a, nxl - 1, jc));
-  }
-  if (!ie && je) {
-    const int ic = ci - 1;
-    if (cj == 0)
-      return lat(v, a, ic, 2) - 3. * (lat(v, a, ic, 1)) + 3. * (lat(v, a, ic, 0));
-    return lat(v, a, ic, nyl - 3) - 3. * (lat(v, a, ic, nyl - 2)) + 3. * (lat(v, a, ic, nyl - 1));
-  }
-  // corners, from the edge values in column 0 / nyl+1 (rows) or row 0 (flct.c:2313-2336)
-  auto edge_col0 = [&](int c) {  // cdata[c][0], 1 <= c <= nxl
-    const int ic = c - 1;
-    return lat(v, a, ic, 2) - 3. * (lat(v, a, ic, 1)) + 3. * (lat(v, a, ic, 0));
-  };
-  auto edge_colN = [&](int c) {  // cdata[c][nyl+1]
-    const int ic = c - 1;
-    return lat(v, a, ic, nyl - 3) - 3. * (lat(v, a, ic, nyl - 2)) + 3. * (lat(v, a, ic, nyl - 1));
-  };
-  auto edge_row0 = [&](int c) {  // cdata[0][c], 1 <= c <= nyl
-    const int jc = c - 1;
-    return lat(v, a, 2, jc) - 3. * (lat(v, a, 1, jc)) + 3. * (lat(v, a, 0, jc));
-  };
-  if (ci == 0 && cj == 0) return 3. * (edge_col0(1)) - 3. * (edge_col0(2)) + edge_col0(3);
-  if (ci == nxl + 1 && cj == 0)
-    return 3. * (edge_col0(nxl)) - 3. * (edge_col0(nxl - 1)) + edge_col0(nxl - 2);
-  if (ci == 0 && cj == nyl + 1)
-    return 3. * (edge_row0(nyl)) - 3. * (edge_row0(nyl - 1)) + edge_row0(nyl - 2);
-  return 3. * (edge_colN(nxl)) - 3. * (edge_colN(nxl - 1)) + edge_colN(nxl - 2);
-}
-
+// interpcc2d over the computed lattice (flct.c:2226-2463)
 __device__ double interp_one(const double* v, const FillArgs& a, double xint, double yint) {
-#pragma clang fp contract(off)
-  const int nxl = a.ixcount, nyl = a.iycount;
-  if ((yint < 0.) || (yint > (double)(nyl - 1)) || ((xint < 0) || (xint > (double)(nxl - 1))))
-    return kXmiss;
-  const int itemp = ((int)xint > 0) ? (int)xint : 0;
-  const int izero = (itemp < (nxl - 2)) ? itemp : nxl - 2;
-  const int jtemp = ((int)yint > 0) ? (int)yint : 0;
-  const int jzero = (jtemp < (nyl - 2)) ? jtemp : nyl - 2;
-  double ftmp = 0.;
-  bool databad = false;
-  for (int ii = -1; ii < 3; ii++) {
-    const double tx = fabs(xint - (double)(izero + ii));
-    const double rx = keys_w(tx);
-    for (int jj = -1; jj < 3; jj++) {
-      const double ty = fabs(yint - (double)(jzero + jj));
-      const double ry = keys_w(ty);
-      const double c = cdata_at(v, a, izero + 1 + ii, jzero + 1 + jj);
-      ftmp = ftmp + c * rx * ry;
-      if (c == kXmiss) databad = true;
-    }
-  }
-  return databad ? kXmiss : ftmp;
+  auto at = [&](int ic, int jc) { return lat(v, a, ic, jc); };
+  return cc_interp(at, a.ixcount, a.iycount, xint, yint, kXmiss);
 }
 
 // skipped pixels get the cubic-convolution fill and vm = 0.5

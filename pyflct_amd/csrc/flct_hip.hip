@@ -14,6 +14,7 @@
 #include "aux_kernels.hpp"
 #include "track_kernels.hpp"
 #include "warp_kernels.hpp"
+#include "interp_cc.hpp"
 #include "whole_kernels.hpp"
 #include "track_launch.hpp"
 
@@ -243,7 +244,9 @@ static const char* path_name(int p) {
 
 // candidate paths of a form and box size (the first is the untuned default
 // unless default_path says otherwise)
-static std::vector<int> path_candidates(int precision, int N, int nx, int ny, int bias) {
+static std::vector<int> path_candidates(int precision, int N, int nx, int ny, int bias,
+                                        int hires) {
+  if (hires) return {PATH_FP64};  // the hires refinement runs in the FP64 kernels
   if (precision == FLCT_HIP_FP32) return {PATH_FP32, PATH_FP64};
   if (precision == FLCT_HIP_SPLIT_BF16) {
     std::vector<int> v;
@@ -341,6 +344,7 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Pla
                                       : default_path(o->precision, N, nx, ny, o->bias, c->bf16_kernel);
   }
   if (pl->path == PATH_BF16_SEG && !seg_supported(N, nx, ny, o->bias)) pl->path = PATH_BF16_PIXEL;
+  if (o->hires) pl->path = PATH_FP64;  // -h: the FP64 kernels carry the hires refinement
   pl->seg = (pl->path == PATH_BF16_SEG);
   if (pl->r.skipon && o->interp) {
     int ixc = 0, iyc = 0;
@@ -451,6 +455,7 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
   ta.nwork = npix;  // upper bound of the list length, for the grid
   ta.nwork_dev = pl.sparse ? counts : nullptr;
   ta.bias = o->bias ? 1 : 0;
+  ta.hires = o->hires ? 1 : 0;
   ta.vx = vx;
   ta.vy = vy;
   ta.vm = vm;
@@ -491,7 +496,10 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
   } else if (pl.path == PATH_FP32) {
     rc = (M <= 12) ? dispatch_track_f32_lo(M, lc, ta) : dispatch_track_f32_hi(M, lc, ta);
   } else {
-    rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, ta) : dispatch_track_f64_hi(M, lc, ta);
+    if (ta.hires)
+      rc = (M <= 12) ? dispatch_track_f64h_lo(M, lc, ta) : dispatch_track_f64h_hi(M, lc, ta);
+    else
+      rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, ta) : dispatch_track_f64_hi(M, lc, ta);
   }
   if (rc) return rc;
   if (ev) HIPCHK(hipEventRecord(ev[2], s));
@@ -605,7 +613,8 @@ static int track_whole(flct_hip_ctx* c, const double* f1, const double* f2, int 
   HIPCHK(hipStreamSynchronize(s));
   double vmaxd;
   std::memcpy(&vmaxd, &hr[0], 8);
-  // peak and 3x3 fit (flct.c:1512-1556, 1639-1640)
+  // peak and 3x3 fit (flct.c:1512-1556, 1639-1640), or the hires refinement
+  // (flct.c:1564-1645)
   int ixmax, iymax;
   const bool absccmax = (vmaxd != 0.);
   if (absccmax) {
@@ -615,8 +624,33 @@ static int track_whole(flct_hip_ctx* c, const double* f1, const double* f2, int 
     ixmax = nx / 2;
     iymax = ny / 2;
   }
-  double shiftsubx = 0., shiftsuby = 0.;
-  if (ixmax > 0 && ixmax < nx - 1 && iymax > 0 && iymax < ny - 1 && absccmax) {
+  double shx = (double)ixmax, shy = (double)iymax;  // peak in surface index units
+  if (o->hires) {
+    // the 6 x 6 window about the integer peak holds every value it reads
+    double win[kHiresWin * kHiresWin] = {0.};
+    const int c0 = std::max(0, iymax - 2), c1 = std::min(ny - 1, iymax + 3);
+    for (int r = std::max(0, ixmax - 2); r <= std::min(nx - 1, ixmax + 3); ++r)
+      HIPCHK(hipMemcpy(&win[(r - ixmax + 2) * kHiresWin + (c0 - iymax + 2)],
+                       sh + (long long)r * ny + c0, (size_t)(c1 - c0 + 1) * 8,
+                       hipMemcpyDeviceToHost));
+    const HiresWindow hw{win, ixmax, iymax};
+    int qmax = 0;
+    double vmx = hires_value(hw, nx, ny, 0);
+    for (int q = 1; q < kHiresN * kHiresN; ++q) {  // maxloc (flct.c:2142-2161)
+      const double v = hires_value(hw, nx, ny, q);
+      if (v > vmx) {
+        vmx = v;
+        qmax = q;
+      }
+    }
+    double nb[9] = {0.};
+    for (int t = 0; t < 9; ++t) {
+      const int ix = qmax / kHiresN + t / 3 - 1, iy = qmax % kHiresN + t % 3 - 1;
+      if (ix >= 0 && ix < kHiresN && iy >= 0 && iy < kHiresN)
+        nb[t] = hires_value(hw, nx, ny, ix * kHiresN + iy);
+    }
+    hires_shift(qmax, nb, ixmax, iymax, &shx, &shy);
+  } else if (ixmax > 0 && ixmax < nx - 1 && iymax > 0 && iymax < ny - 1 && absccmax) {
     double a[3][3];
     for (int r = 0; r < 3; ++r)
       HIPCHK(hipMemcpy(a[r], sh + (long long)(ixmax - 1 + r) * ny + (iymax - 1), 3 * 8,
@@ -628,12 +662,14 @@ static int track_whole(flct_hip_ctx* c, const double* f1, const double* f2, int 
       const double fxx = (a[2][1] + a[0][1] - 2. * (a[1][1]));
       const double fyy = (a[1][2] + a[1][0] - 2. * (a[1][1]));
       const double fxy = 0.25 * (a[2][2] + a[0][0] - a[2][0] - a[0][2]);
-      shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
-      shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
+      const double shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
+      const double shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
+      shx = (double)ixmax + shiftsubx;
+      shy = (double)iymax + shiftsuby;
     }
   }
-  const double shiftx = ((double)ixmax + shiftsubx) - (double)(nx / 2);
-  const double shifty = ((double)iymax + shiftsuby) - (double)(ny / 2);
+  const double shiftx = shx - (double)(nx / 2);
+  const double shifty = shy - (double)(ny / 2);
   const double deltinv = 1. / o->deltat;
   out[0] = shifty * deltinv * o->deltas;  // transp = 1 (flct.c:900-905)
   out[1] = shiftx * deltinv * o->deltas;
@@ -781,7 +817,7 @@ extern "C" int flct_hip_autotune(flct_hip_ctx* c, const double* f1, const double
     return rc;
   int best = -1;
   double best_ms = 0.;
-  for (int p : path_candidates(o->precision, pl.N, nx, ny, o->bias)) {
+  for (int p : path_candidates(o->precision, pl.N, nx, ny, o->bias, o->hires)) {
     c->force_path = p;
     double t = 0.;
     for (int rep = 0; rep < 3 && !rc; ++rep) {  // first run warms up

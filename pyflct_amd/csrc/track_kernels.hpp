@@ -16,7 +16,11 @@
 //  3. column c2r along k for two columns per transform, |.|, the fftshift
 //     by (N/2, N/2) as index arithmetic, the first-strict-maximum argmax
 //     (flct.c:1512-1534, 2142-2161), the 3x3 Taylor fit (flct.c:1536-1556,
-//     1639-1640) and vx, vy (flct.c:901-905).
+//     1639-1640) and vx, vy (flct.c:901-905).  With -h (HIRES, double) the
+//     fit is replaced by the hires refinement of cross_cor: the surface is
+//     interpolated by cubic convolution on a 21 x 21 grid of 0.1 px about
+//     the integer peak, and the grid maximum gets the 3 x 3 fit
+//     (flct.c:1564-1645).
 //
 // With bias correction on, the pixel is tracked a second time with the
 // Gaussian windows of image 1 and image 2 centred at -d/2 and +d/2 about the
@@ -24,6 +28,7 @@
 #pragma once
 
 #include "fft_quad.hpp"
+#include "interp_cc.hpp"
 
 namespace flct {
 
@@ -46,6 +51,7 @@ struct TrackArgs {
   int* flag_list;                    // pixels to re-evaluate in FP64, or nullptr
   unsigned int* flag_count;
   int bias;
+  int hires;  // -h: run the HIRES instantiation (cubic-convolution peak on a 0.1 px grid)
   double* vx;
   double* vy;
   double* vm;
@@ -136,8 +142,11 @@ __device__ __forceinline__ cplx<T> cross_power(cplx<T> A, cplx<T> B, T norm) {
   return {(A.re * B.re + A.im * B.im) * norm, (A.re * B.im - A.im * B.re) * norm};
 }
 
-template <int M, class T>
+// HIRES (double only): the -h peak step replaces the 3 x 3 fit; a separate
+// instantiation, so the plain kernels keep their register allocation
+template <int M, class T, bool HIRES = false>
 __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs a) {
+  static_assert(!HIRES || sizeof(T) == 8, "the hires step runs in the double kernels");
   using Cfg = TrackCfg<M, T>;
   constexpr int N = 4 * M;
   extern __shared__ __align__(16) unsigned char lds[];
@@ -473,7 +482,61 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
         ixmax = (int)(loc / N);
         iymax = (int)(loc % N);
       }
-      const bool fit = nonzero && ixmax > 0 && ixmax < N - 1 && iymax > 0 && iymax < N - 1;
+      bool fit = nonzero && ixmax > 0 && ixmax < N - 1 && iymax > 0 && iymax < N - 1;
+      int hq = -1;  // hires grid maximum
+      if constexpr (HIRES) {
+        {
+          fit = false;
+          // 6 x 6 window of the surface about the integer peak and the 6 x 6
+          // cdata block the grid reads, in Z (whose last reads were before
+          // the barriers above)
+          double* hw = reinterpret_cast<double*>(Z);
+          double* hc = hw + kHiresWin * kHiresWin;
+          unsigned long long* hkey = reinterpret_cast<unsigned long long*>(hc + kHiresWin * kHiresWin);
+          unsigned int* hloc = reinterpret_cast<unsigned int*>(hkey + 1);
+          if (active) {
+            static_for<M>([&](auto S) {
+              const int di = (ib + S + N / 2) % N - ixmax;
+              if (di >= -2 && di <= 3) {
+                const int da = sa - iymax, db = sb - iymax;
+                if (da >= -2 && da <= 3) hw[(di + 2) * kHiresWin + (da + 2)] = (double)va[S];
+                if (db >= -2 && db <= 3) hw[(di + 2) * kHiresWin + (db + 2)] = (double)vb[S];
+              }
+            });
+            if (stid == 0) {
+              *hkey = 0ull;
+              *hloc = 0xffffffffu;
+            }
+          }
+          __syncthreads();
+          if (active && stid < kHiresWin * kHiresWin)
+            hires_cdata_entry(HiresWindow{hw, ixmax, iymax}, N, N, stid, hc);
+          __syncthreads();
+          unsigned long long kbest = 0ull;
+          int qbest = 0x7fffffff;
+          if (active) {
+#pragma unroll 1
+            for (int q = stid; q < kHiresN * kHiresN; q += Cfg::kSlotThreads) {
+              const unsigned long long k = hires_key(hires_value_cp(hc, N, N, ixmax, iymax, q));
+              if (k > kbest) {
+                kbest = k;
+                qbest = q;
+              }
+            }
+            atomicMax(hkey, kbest);
+          }
+          __syncthreads();
+          if (active && kbest == *hkey) atomicMin(hloc, (unsigned)qbest);
+          __syncthreads();
+          hq = (int)*hloc;
+          if (active && stid < 9) {
+            // 3 x 3 neighbourhood of the grid maximum, where it is interior
+            const int ix = hq / kHiresN + stid / 3 - 1, iy = hq % kHiresN + stid % 3 - 1;
+            if (ix >= 0 && ix < kHiresN && iy >= 0 && iy < kHiresN)
+              nb[stid] = hires_value_cp(hc, N, N, ixmax, iymax, ix * kHiresN + iy);
+          }
+        }
+      }
       if (active && fit) {
         static_for<M>([&](auto S) {
           const int si = (ib + S + N / 2) % N;
@@ -489,7 +552,12 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
       if (active && stid == 0) {
 #pragma clang fp contract(off)
         double shiftsubx = 0., shiftsuby = 0.;
-        if (fit) {
+        if (hq >= 0) {
+          double shx, shy;
+          hires_shift(hq, nb, ixmax, iymax, &shx, &shy);
+          res[0] = shx - (double)(N / 2);
+          res[1] = shy - (double)(N / 2);
+        } else if (fit) {
           // nb[(dx+1)*3 + (dy+1)] = surface at (ixmax+dx, iymax+dy)
           const double c00 = nb[4];
           const double fx = 0.5 * (nb[7] - nb[1]);
@@ -500,8 +568,10 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
           shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
           shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
         }
-        res[0] = ((double)ixmax + shiftsubx) - (double)(N / 2);
-        res[1] = ((double)iymax + shiftsuby) - (double)(N / 2);
+        if (hq < 0) {
+          res[0] = ((double)ixmax + shiftsubx) - (double)(N / 2);
+          res[1] = ((double)iymax + shiftsuby) - (double)(N / 2);
+        }
       }
       __syncthreads();
       shiftx = res[0];

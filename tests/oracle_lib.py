@@ -59,12 +59,13 @@ class using:
 def _bind(path):
     L = ctypes.CDLL(path)
     i, d = ctypes.c_int, ctypes.c_double
-    L.oracle_flct.argtypes = [_dp, _dp, i, i, d, d, d, d, i, i, i, i, i, i, d, i,
+    L.oracle_flct.argtypes = [_dp, _dp, i, i, d, d, d, d, i, i, i, i, i, i, d, i, i,
                               _dp, _dp, _dp]
     L.oracle_flct_pixels.argtypes = [_dp, _dp, i, i, d, d, d, d, i, i, i, i, i,
-                                     d, i, _lp, ctypes.c_longlong, _dp, _dp, _dp]
+                                     d, i, i, _lp, ctypes.c_longlong, _dp, _dp, _dp]
     L.oracle_cross_cor.argtypes = [_dp, _dp, i, i, i, d, _dp, _dp, _dp]
     L.oracle_peak_fit.argtypes = [_dp, i, i, _dp, _dp]
+    L.oracle_peak_hires.argtypes = [_dp, i, i, _dp, _dp]
     L.oracle_interpcc2d.argtypes = [_dp, d, i, i, _dp, i, _dp, i, _dp]
     L.oracle_fft2.argtypes = [_dp, i, i, i]
     L.oracle_make_freq.argtypes = [_dp, i]
@@ -81,7 +82,7 @@ def _f64(a):
 
 
 def flct(f1, f2, deltat, deltas, sigma, thresh=0.0, thresh_abs=False, skip=0,
-         poff=0, qoff=0, interp=False, kr=None, bias=False):
+         poff=0, qoff=0, interp=False, kr=None, bias=False, hires=False):
     """Whole flct run on C-space arrays; returns (vx, vy, vm), 1 x 1 for
     sigma = 0."""
     f1, f2 = _f64(f1), _f64(f2)
@@ -90,7 +91,7 @@ def flct(f1, f2, deltat, deltas, sigma, thresh=0.0, thresh_abs=False, skip=0,
     vx, vy, vm = (np.empty(shape) for _ in range(3))
     rc = lib().oracle_flct(_ptr(f1), _ptr(f2), nx, ny, deltat, deltas, sigma, thresh,
                            int(thresh_abs), skip, poff, qoff, int(interp),
-                           int(kr is not None), float(kr or 0.0), int(bias),
+                           int(kr is not None), float(kr or 0.0), int(bias), int(hires),
                            _ptr(vx), _ptr(vy), _ptr(vm))
     if rc:
         raise RuntimeError(f"oracle_flct status {rc}")
@@ -98,7 +99,7 @@ def flct(f1, f2, deltat, deltas, sigma, thresh=0.0, thresh_abs=False, skip=0,
 
 
 def flct_pixels(f1, f2, deltat, deltas, sigma, pix, thresh=0.0, thresh_abs=False,
-                skip=0, poff=0, qoff=0, kr=None, bias=False):
+                skip=0, poff=0, qoff=0, kr=None, bias=False, hires=False):
     """Track only the C-space linear pixel indices ``pix`` (no skip fill)."""
     f1, f2 = _f64(f1), _f64(f2)
     nx, ny = f1.shape
@@ -108,7 +109,7 @@ def flct_pixels(f1, f2, deltat, deltas, sigma, pix, thresh=0.0, thresh_abs=False
     rc = lib().oracle_flct_pixels(_ptr(f1), _ptr(f2), nx, ny, deltat, deltas, sigma,
                                   thresh, int(thresh_abs), skip, poff, qoff,
                                   int(kr is not None), float(kr or 0.0), int(bias),
-                                  pix.ctypes.data_as(_lp), n, _ptr(vx), _ptr(vy),
+                                  int(hires), pix.ctypes.data_as(_lp), n, _ptr(vx), _ptr(vy),
                                   _ptr(vm))
     if rc:
         raise RuntimeError(f"oracle_flct_pixels status {rc}")
@@ -131,6 +132,14 @@ def peak_fit(surf):
     sx, sy = ctypes.c_double(), ctypes.c_double()
     lib().oracle_peak_fit(_ptr(surf), surf.shape[0], surf.shape[1], ctypes.byref(sx),
                           ctypes.byref(sy))
+    return sx.value, sy.value
+
+
+def peak_hires(surf):
+    surf = _f64(surf)
+    sx, sy = ctypes.c_double(), ctypes.c_double()
+    lib().oracle_peak_hires(_ptr(surf), surf.shape[0], surf.shape[1], ctypes.byref(sx),
+                            ctypes.byref(sy))
     return sx.value, sy.value
 
 

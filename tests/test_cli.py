@@ -43,8 +43,6 @@ def test_usage_and_fatal_option_paths(tmp_path):
     vcimage.write(tiny, np.ones((5, 5)), np.ones((5, 5)))
     rc, out = run(FLCT, str(tiny), "/tmp/x.dat", "1", "1", "5")
     assert rc == 1 and "illegal box size" in out  # flct.c:575-579
-    rc, out = run(FLCT, os.path.join(GOLD, "hashgauss.dat"), "/tmp/x.dat", "1", "1", "5", "-h")
-    assert rc == 1 and "-h" in out
     rc, out = run(WARP, "a")
     assert rc == 1 and "usage" in out
 
@@ -85,6 +83,21 @@ def test_flct_cli_options_match_api(tmp_path, ctx):
     assert rc == 0, text
     r = vcimage.read3(out)
     g = ctx.track(g1, g2, 2.0, 0.5, 8.0, thresh=0.05, skip=3, poff=-1, qoff=1, interp=True, kr=0.6)
+    for a, b in zip(r, g):
+        assert np.array_equal(a, f32(b))
+
+
+@pytest.mark.gpu
+def test_flct_cli_hires_matches_api(tmp_path, ctx, golden):
+    """flct hashgauss.dat out.dat 1 1 5 -k 0.5 -h: the -h peak step through
+    the program equals the API's."""
+    from pyflct_amd import vcimage
+    out = tmp_path / "h.dat"
+    rc, text = run(FLCT, os.path.join(GOLD, "hashgauss.dat"), str(out), "1", "1", "5", "-k", "0.5", "-h")
+    assert rc == 0, text
+    assert "hires option turned on" in text
+    r = vcimage.read3(out)
+    g = ctx.track(golden["f1"], golden["f2"], 1.0, 1.0, 5.0, kr=0.5, hires=True)
     for a, b in zip(r, g):
         assert np.array_equal(a, f32(b))
 

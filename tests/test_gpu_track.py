@@ -268,3 +268,47 @@ def test_sigma0_ignores_threshold_skip_and_takes_device_arrays(ctx):
     z = np.zeros((33, 48))
     assert [float(v[0, 0]) for v in ctx.track(z, z, 1.0, 1.0, 0.0)] == [0.0, 0.0, 1.0]
     assert_parity(ctx.track(z, z, 1.0, 1.0, 0.0), O.flct(z, z, 1.0, 1.0, 0.0))
+
+
+# ------------------------------------------------------------------ -h
+@pytest.mark.parametrize("sigma,kw", [(4.0, {}), (5.0, {"kr": 0.5}), (6.0, {"bias": True}),
+                                      (8.0, {"thresh": 0.3}), (10.0, {"skip": 3, "poff": 1, "interp": True}),
+                                      (15.0, {})])
+def test_hires_peak(ctx, sigma, kw):
+    """-h: the cubic-convolution peak on the 0.1 px grid (flct.c:1357-1368,
+    1564-1645) in the FP64 kernels, against the oracle on every pixel."""
+    rng = np.random.default_rng(int(sigma * 10))
+    nx, ny = (100, 110) if sigma < 15 else (150, 140)
+    f1, f2 = smooth_pair(rng, nx, ny, shift=(0.45, -0.7), width=2.5)
+    g = ctx.track(f1, f2, 1.0, 1.0, sigma, hires=True, **kw)
+    assert ctx.last_path() == "fp64-fft"
+    o = O.flct(f1, f2, 1.0, 1.0, sigma, hires=True, **kw)
+    assert_parity(g, o, where=f"hires sigma={sigma} {kw}")
+    plain = ctx.track(f1, f2, 1.0, 1.0, sigma, **kw)
+    assert not np.array_equal(g[0], plain[0])
+
+
+def test_hires_zero_images_and_other_forms(ctx):
+    """All-zero boxes give the grid-corner shift (-1, -1) under -h, as in the
+    reference's rules; FP32 and split-BF16 requests with -h run the FP64
+    kernels and return the FP64 result."""
+    z = np.zeros((40, 44))
+    g = ctx.track(z, z, 2.0, 3.0, 5.0, hires=True)
+    assert np.all(g[0] == -1.5) and np.all(g[1] == -1.5) and np.all(g[2] == 1.0)
+    assert_parity(g, O.flct(z, z, 2.0, 3.0, 5.0, hires=True))
+    rng = np.random.default_rng(77)
+    f1, f2 = smooth_pair(rng, 128, 128, shift=(-0.3, 0.55))
+    ref = ctx.track(f1, f2, 1.0, 1.0, 12.0, hires=True)
+    for form in ("fp32", "split_bf16"):
+        got = ctx.track(f1, f2, 1.0, 1.0, 12.0, hires=True, precision=form)
+        assert ctx.last_path() == "fp64-fft"
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b), form
+
+
+@pytest.mark.parametrize("nx,ny,kw", [(64, 80, {}), (101, 97, {"kr": 0.5})])
+def test_hires_sigma0(ctx, nx, ny, kw):
+    f1, f2 = _whole_pair(nx, ny, seed=nx * ny)
+    g = ctx.track(f1, f2, 1.0, 1.0, 0.0, hires=True, **kw)
+    o = O.flct(f1, f2, 1.0, 1.0, 0.0, hires=True, **kw)
+    assert_parity(g, o, where=f"hires sigma=0 {nx}x{ny}")

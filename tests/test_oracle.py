@@ -96,6 +96,62 @@ def test_peak_edge_and_zero_rules():
     assert np.floor(sx + n // 2 + 0.5) == 3 and np.floor(sy + n // 2 + 0.5) == 4
 
 
+def test_hires_peak_on_sampled_quadratic():
+    """-h (flct.c:1564-1645): cubic convolution reproduces a quadratic
+    exactly away from the border, so the 0.1 px grid holds the quadratic and
+    the 3 x 3 fit there, scaled to pixels, returns its exact vertex; the
+    vertex is placed between grid points so that the fit's scale matters."""
+    n = 16
+    for x0, y0 in ((8.33, 7.57), (7.96, 8.04), (6.71, 9.38)):
+        i, j = np.mgrid[0:n, 0:n].astype(float)
+        dx, dy = i - x0, j - y0
+        surf = 100.0 - (1.3 * dx * dx + 0.4 * dx * dy + 0.9 * dy * dy)
+        sx, sy = O.peak_hires(surf)
+        assert abs(sx - (x0 - n // 2)) < 1e-10 and abs(sy - (y0 - n // 2)) < 1e-10, (x0, y0)
+        # the coarse fit agrees on this surface (both exact for quadratics)
+        cx, cy = O.peak_fit(surf)
+        assert abs(cx - sx) < 1e-10 and abs(cy - sy) < 1e-10
+
+
+def test_hires_zero_and_edge_rules():
+    """The grid is interpolated with xmiss = 0 (flct.c:1589-1590): grid points
+    outside the surface, and stencils touching a zero, read 0; an all-zero
+    grid has its first point as maximum, so the shift is the grid corner
+    (-1, -1) about the integer peak, which is the centre when the surface is
+    zero (flct.c:1518-1530)."""
+    n = 12
+    assert O.peak_hires(np.zeros((n, n))) == (-1.0, -1.0)
+    s = np.zeros((n, n))
+    s[0, 5] = 1.0  # a lone edge peak: every stencil touches a zero
+    assert O.peak_hires(s) == (-1.0 - n // 2, 5.0 - 1.0 - n // 2)
+    # a smooth peak: the grid maximum is interior and near the true centre
+    i, j = np.mgrid[0:n, 0:n].astype(float)
+    g = np.exp(-((i - 5.37) ** 2 + (j - 6.81) ** 2) / 8.0) + 0.1
+    sx, sy = O.peak_hires(g)
+    assert abs(sx - (5.37 - n // 2)) < 0.05 and abs(sy - (6.81 - n // 2)) < 0.05
+
+
+def test_flct_hires_tracks_a_uniform_shift():
+    """flct with -h on a uniformly shifted image: the velocities follow the
+    shift like the plain fit's (both carry the windowed correlation's
+    underestimate at sigma = 6), and differ from it (the peak step is really
+    replaced)."""
+    rng = np.random.default_rng(21)
+    from scipy.ndimage import gaussian_filter
+    g = gaussian_filter(rng.standard_normal((48, 52)), 2.0, mode="wrap")
+    kx = np.fft.fftfreq(48)[:, None]
+    ky = np.fft.fftfreq(52)[None, :]
+    h = np.fft.ifft2(np.fft.fft2(g) * np.exp(-2j * np.pi * (kx * 0.4 + ky * -0.25))).real
+    hv = O.flct(g, h, 1.0, 1.0, 6.0, hires=True)
+    pv = O.flct(g, h, 1.0, 1.0, 6.0)
+    sl = (slice(12, 36), slice(12, 40))
+    for k, true in ((0, -0.25), (1, 0.4)):
+        mh, mp = np.median(hv[k][sl]), np.median(pv[k][sl])
+        assert abs(mh - mp) < 0.05 and 0.6 * abs(true) < abs(mh) <= abs(true) and mh * true > 0
+        assert np.max(np.abs(hv[k][sl] - pv[k][sl])) < 0.1
+    assert not np.array_equal(hv[0], pv[0])
+
+
 def test_interpcc2d_reproduces_quadratics_and_border_rule():
     """Keys cubic convolution (a = -0.5) is exact for quadratics in the
     interior; the one-cell border uses f[2] - 3 f[1] + 3 f[0] (flct.c:2294)."""

@@ -20,6 +20,7 @@ PLANTS = {
     "PLANT_NO_SWAP": [T.test_flct_reproduces_reference_fixture,
                       T.test_uniform_shift_recovers_known_velocity],
     "PLANT_NYQUIST_NEG": [T.test_make_freq_closed_form],
+    "PLANT_HIRES_SCALE": [T.test_hires_peak_on_sampled_quadratic],
 }
 
 
