@@ -169,6 +169,20 @@ int flct_hip_autotune(flct_hip_ctx *ctx, const double *d_f1, const double *d_f2,
 int flct_hip_warp(flct_hip_ctx *ctx, const double *arr, const double *delx,
                   const double *dely, double *outarr, int nx, int ny, int transp, int quiet);
 
+/* Evaluation method of flct_hip_warp: the reference's direct sum of the
+ * Fourier series at every pixel (O(nx ny) work per pixel, warp.c:960-991),
+ * or the same series as a type-2 non-uniform FFT (a 2x oversampled grid,
+ * 16 x 16 interpolation taps per pixel), which differs from the direct sum
+ * by about 1e-13 of max|image|.  AUTO (the default) takes the NUFFT for
+ * 16 <= nx, ny <= 4096 and the direct sum otherwise.  No counterpart in the
+ * reference. */
+#define FLCT_HIP_WARP_AUTO 0
+#define FLCT_HIP_WARP_DIRECT 1
+#define FLCT_HIP_WARP_NUFFT 2
+int flct_hip_set_warp_method(flct_hip_ctx *ctx, int method);
+/* Method the last warp call ran (FLCT_HIP_WARP_DIRECT or _NUFFT; 0 before any). */
+int flct_hip_last_warp_method(flct_hip_ctx *ctx);
+
 /* Device-pointer form of flct_hip_warp. */
 int flct_hip_warp_device(flct_hip_ctx *ctx, const double *d_arr, const double *d_delx,
                          const double *d_dely, double *d_out, int nx, int ny, int transp);

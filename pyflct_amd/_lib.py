@@ -48,6 +48,8 @@ EXPORTS = (
     "flct_hip_autotune",
     "flct_hip_warp",
     "flct_hip_warp_device",
+    "flct_hip_set_warp_method",
+    "flct_hip_last_warp_method",
     "flct_hip_shift",
     "flct_hip_malloc",
     "flct_hip_free",
@@ -130,6 +132,8 @@ def lib():
         L.flct_hip_autotune.argtypes = [vp, vp, vp, i, i, ctypes.POINTER(Opts)]
         L.flct_hip_warp.argtypes = [vp, _dp, _dp, _dp, _dp, i, i, i, i]
         L.flct_hip_warp_device.argtypes = [vp, vp, vp, vp, vp, i, i, i]
+        L.flct_hip_set_warp_method.argtypes = [vp, i]
+        L.flct_hip_last_warp_method.argtypes = [vp]
         L.flct_hip_shift.argtypes = [vp, _dp, d, d, _dp, i, i, i, i]
         L.flct_hip_malloc.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_longlong]
         L.flct_hip_free.argtypes = [vp, vp]

@@ -252,6 +252,18 @@ class Context:
                                        ny, int(bool(transp)), 1), "flct_hip_warp")
         return out
 
+    WARP_METHODS = {"auto": 0, "direct": 1, "nufft": 2}
+
+    def set_warp_method(self, method):
+        """"auto" (NUFFT for 16 <= nx, ny <= 4096), "direct" (the reference's
+        per-pixel sum) or "nufft" (flct_hip_set_warp_method)."""
+        check(_lib.lib().flct_hip_set_warp_method(self._h, self.WARP_METHODS[method]),
+              "flct_hip_set_warp_method")
+
+    def last_warp_method(self):
+        m = _lib.lib().flct_hip_last_warp_method(self._h)
+        return {1: "direct", 2: "nufft"}.get(m, "none")
+
     def warp_device(self, d_arr, d_delx, d_dely, d_out, nx, ny, transp=True):
         """warp_frac2d on device arrays (flct_hip_warp_device)."""
         check(_lib.lib().flct_hip_warp_device(self._h, d_arr, d_delx, d_dely, d_out, int(nx),

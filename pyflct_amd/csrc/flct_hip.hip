@@ -16,6 +16,7 @@
 #include "warp_kernels.hpp"
 #include "interp_cc.hpp"
 #include "whole_kernels.hpp"
+#include "nufft_kernels.hpp"
 #include "track_launch.hpp"
 
 using namespace flct;
@@ -37,7 +38,7 @@ struct flct_hip_ctx {
   size_t lds_per_block = 0;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   DevBuf f1, f2, vx, vy, vm, list, count, gw, filt, absmax, wtmp, wF, wtw1, wtw2, w3, w4, w5,
-      hs1, hs2, mean1, mean2, flags, fcount, segtab, elist, ecount;
+      hs1, hs2, mean1, mean2, flags, fcount, segtab, elist, ecount, nuT, nuR, nuTab;
   // batch: own scratch, table arena and counters, and the cached graph
   DevBuf bscr[7], barena, bcount;
   hipGraph_t graph = nullptr;
@@ -57,6 +58,8 @@ struct flct_hip_ctx {
   double last_total_ms = 0.;
   double last_main_ms = 0.;    // the tracking kernel alone
   double last_refine_ms = 0.;  // FP64 re-evaluation of flagged pixels (split-BF16 form)
+  int warp_method = FLCT_HIP_WARP_AUTO;
+  int last_warp_method = 0;
 };
 
 #define HIPCHK(x)                              \
@@ -994,6 +997,133 @@ static void root_table(std::vector<double>& t, int n) {
 
 static unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Gauss-Legendre nodes and weights on [-1, 1] (Newton on P_n)
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.resize(n);
+  w.resize(n);
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1., p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2. * k - 1.) * z * p1 - (k - 1.) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[i] = z;
+    w[i] = 2. / ((1. - z * z) * dp * dp);
+  }
+}
+
+// 1 / phihat(2 pi k / M) of the warp's spreading kernel for the modes
+// k = make_freq(a, n), a < n: phihat(xi) = 2 int_0^{w/2} phi(z) cos(xi z) dz
+static void nufft_deconv(int n, int M, std::vector<double>& d) {
+  static std::vector<double> gx, gw;
+  if (gx.empty()) gauss_legendre(128, gx, gw);
+  const double half = 0.5 * kNuW, beta = kNuBetaPerW * kNuW;
+  std::vector<double> z(gx.size()), pw(gx.size());
+  for (size_t q = 0; q < gx.size(); ++q) {
+    z[q] = 0.5 * half * (gx[q] + 1.);
+    const double u = 2. * z[q] / kNuW;
+    pw[q] = std::exp(beta * (std::sqrt(std::max(0., 1. - u * u)) - 1.)) * gw[q] * 0.5 * half;
+  }
+  d.resize(n);
+  const int n21 = n / 2 - 1;
+  for (int a = 0; a < n; ++a) {
+    double k;  // make_freq (warp.c:316-360)
+    if (a <= n21) k = a;
+    else if (n % 2) k = (a == n21 + 1) ? n / 2 : (a == n21 + 2) ? -(n / 2) : -(n21 - (a - (n21 + 3)));
+    else k = (a == n21 + 1) ? n / 2 : -(n21 - (a - (n21 + 2)));
+    const double xi = 2. * M_PI * k / (double)M;
+    double acc = 0.;
+    for (size_t q = 0; q < z.size(); ++q) acc += pw[q] * std::cos(xi * z[q]);
+    d[a] = 1. / (2. * acc);
+  }
+}
+
+static int log2_grid(int n) {  // smallest power of two >= 2 n
+  int l = 0;
+  while ((1 << l) < 2 * n) ++l;
+  return l;
+}
+
+static bool nufft_fits(int nx, int ny) {
+  return nx >= 16 && ny >= 16 && log2_grid(nx) <= kNuMaxLog2 && log2_grid(ny) <= kNuMaxLog2;
+}
+
+// warp_frac2d as a type-2 NUFFT on the half spectrum c->wF (already formed)
+static int warp_nufft(flct_hip_ctx* c, const double* delx, const double* dely, double* out, int nx,
+                      int ny, int transp) {
+  hipStream_t s = c->stream;
+  const int lmx = log2_grid(nx), lmy = log2_grid(ny);
+  const int Mx = 1 << lmx, My = 1 << lmy;
+  std::vector<double> dcx, dcy;
+  nufft_deconv(nx, Mx, dcx);
+  nufft_deconv(ny, My, dcy);
+  // table block: dcx | dcy | twx | twy (doubles)
+  std::vector<double> tab;
+  tab.reserve(nx + ny + Mx + My);
+  tab.insert(tab.end(), dcx.begin(), dcx.end());
+  tab.insert(tab.end(), dcy.begin(), dcy.end());
+  const size_t otx = tab.size();
+  for (int e = 0; e < Mx / 2; ++e) {
+    tab.push_back(std::cos(2. * M_PI * e / Mx));
+    tab.push_back(std::sin(2. * M_PI * e / Mx));
+  }
+  const size_t oty = tab.size();
+  for (int e = 0; e < My / 2; ++e) {
+    tab.push_back(std::cos(2. * M_PI * e / My));
+    tab.push_back(std::sin(2. * M_PI * e / My));
+  }
+  int rc;
+  if ((rc = ensure(c->nuTab, tab.size() * 8)) || (rc = ensure(c->nuT, (size_t)nx * My * 16)) ||
+      (rc = ensure(c->nuR, (size_t)My * Mx * 8)))
+    return rc;
+  HIPCHK(hipMemcpyAsync(c->nuTab.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, s));
+  const double* tb = (const double*)c->nuTab.p;
+  NufftArgs q;
+  q.F = (const double*)c->wF.p;
+  q.nx = nx;
+  q.ny = ny;
+  q.nb = ny / 2 + 1;
+  q.lmx = lmx;
+  q.lmy = lmy;
+  q.dcx = tb;
+  q.dcy = tb + nx;
+  q.twx = reinterpret_cast<const double2*>(tb + otx);
+  q.twy = reinterpret_cast<const double2*>(tb + oty);
+  q.T = (double2*)c->nuT.p;
+  q.Rt = (double*)c->nuR.p;
+  q.delx = delx;
+  q.dely = dely;
+  q.transp = transp;
+  q.out = out;
+  const size_t ldsr = (size_t)My * 16, ldsc = (size_t)Mx * 16;
+  if (ldsr > c->lds_per_block || ldsc > c->lds_per_block) return FLCT_HIP_E_UNSUPPORTED;
+  (void)hipFuncSetAttribute((const void*)k_nufft_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)ldsr);
+  (void)hipFuncSetAttribute((const void*)k_nufft_cols, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)ldsc);
+  hipLaunchKernelGGL(k_nufft_rows, dim3(nx), dim3(kNuThreads), ldsr, s, q);
+  hipLaunchKernelGGL(k_nufft_cols, dim3(My), dim3(kNuThreads), ldsc, s, q);
+  hipLaunchKernelGGL(k_nufft_gather, dim3(nblk((long long)nx * ny, 256)), dim3(256), 0, s, q);
+  HIPCHK(hipGetLastError());
+  return FLCT_HIP_OK;
+}
+
+extern "C" int flct_hip_set_warp_method(flct_hip_ctx* c, int method) {
+  if (!c || method < FLCT_HIP_WARP_AUTO || method > FLCT_HIP_WARP_NUFFT) return FLCT_HIP_E_ARG;
+  c->warp_method = method;
+  return FLCT_HIP_OK;
+}
+
+extern "C" int flct_hip_last_warp_method(flct_hip_ctx* c) { return c ? c->last_warp_method : 0; }
+
 extern "C" int flct_hip_warp_device(flct_hip_ctx* c, const double* arr, const double* delx,
                                     const double* dely, double* out, int nx, int ny, int transp) {
   if (!c || !arr || !delx || !dely || !out || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
@@ -1021,6 +1151,19 @@ extern "C" int flct_hip_warp_device(flct_hip_ctx* c, const double* arr, const do
                      1LL, (long long)nb, -1, (const double*)c->wtw2.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->ev[1], s));
+  const bool nufft = (c->warp_method == FLCT_HIP_WARP_NUFFT) ||
+                     (c->warp_method == FLCT_HIP_WARP_AUTO && nufft_fits(nx, ny));
+  if (nufft) {
+    if (!nufft_fits(nx, ny)) return FLCT_HIP_E_UNSUPPORTED;
+    if ((rc = warp_nufft(c, delx, dely, out, nx, ny, transp))) return rc;
+    HIPCHK(hipEventRecord(c->ev[2], s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->last_warp_method = FLCT_HIP_WARP_NUFFT;
+    c->last_kernel_ms = ev_ms(c->ev[1], c->ev[2]);
+    c->last_total_ms = ev_ms(c->ev[0], c->ev[2]);
+    return FLCT_HIP_OK;
+  }
+  c->last_warp_method = FLCT_HIP_WARP_DIRECT;
   WarpArgs w;
   w.F = (const double*)c->wF.p;
   w.nx = nx;

@@ -62,3 +62,74 @@ def test_warp_large_properties(ctx):
     assert np.max(np.abs(w - a)) < 1e-9 * np.max(np.abs(a))
     w = ctx.warp(a, z + 5.0, z + 1.0)
     assert np.max(np.abs(w - np.roll(np.roll(a, 1, axis=0), 5, axis=1))) < 1e-9 * np.max(np.abs(a))
+
+
+# ------------------------------------------------ direct sum and NUFFT
+@pytest.mark.parametrize("shape", [(37, 50), (64, 64), (48, 31), (96, 80), (129, 100)])
+def test_warp_nufft_matches_direct_sum(ctx, shape):
+    """The NUFFT evaluation of warp_frac2d's Fourier series against the
+    reference's direct per-pixel sum (GPU) and the oracle, both transposes."""
+    rng = np.random.default_rng(100 + sum(shape))
+    a = rng.standard_normal(shape) * 10.0
+    dx = rng.standard_normal(shape) * 2.0
+    dy = rng.standard_normal(shape) * 2.0
+    try:
+        for transp in (1, 0):
+            ctx.set_warp_method("direct")
+            wd = ctx.warp(a, dx, dy, transp=bool(transp))
+            assert ctx.last_warp_method() == "direct"
+            ctx.set_warp_method("nufft")
+            wn = ctx.warp(a, dx, dy, transp=bool(transp))
+            assert ctx.last_warp_method() == "nufft"
+            scale = np.max(np.abs(a))
+            assert np.max(np.abs(wn - wd)) < 1e-11 * scale, (shape, transp)
+            wo = O.warp_frac2d(a, dx, dy, transp=transp)
+            assert np.max(np.abs(wn - wo)) < 1e-11 * scale
+    finally:
+        ctx.set_warp_method("auto")
+
+
+def test_warp_method_selection(ctx):
+    from pyflct_amd._lib import FlctHipError
+    rng = np.random.default_rng(9)
+    small = rng.standard_normal((12, 40))
+    z = np.zeros_like(small)
+    ctx.warp(small, z, z)
+    assert ctx.last_warp_method() == "direct"  # below 16 rows: the direct sum
+    big = rng.standard_normal((40, 40))
+    ctx.warp(big, np.zeros_like(big), np.zeros_like(big))
+    assert ctx.last_warp_method() == "nufft"
+    try:
+        ctx.set_warp_method("nufft")
+        with pytest.raises(FlctHipError):
+            ctx.warp(small, z, z)
+    finally:
+        ctx.set_warp_method("auto")
+
+
+def test_warp_nufft_large_fields(ctx):
+    """2048 x 1536 (NUFFT): an integer field is a roll and a uniform field is
+    shift_frac2d; at 1024 x 768 a smooth field agrees with the direct sum."""
+    rng = np.random.default_rng(5)
+    nx, ny = 2048, 1536
+    a = rng.laplace(0, 50, (nx, ny))
+    z = np.zeros_like(a)
+    scale = np.max(np.abs(a))
+    w = ctx.warp(a, z - 7.0, z + 3.0)
+    assert ctx.last_warp_method() == "nufft"
+    assert np.max(np.abs(w - np.roll(np.roll(a, 3, axis=0), -7, axis=1))) < 1e-10 * scale
+    w = ctx.warp(a, z + 0.37, z - 1.21)
+    assert np.max(np.abs(w - ctx.shift(a, 0.37, -1.21))) < 1e-10 * scale
+    nx, ny = 1024, 768
+    a = a[:nx, :ny].copy()
+    scale = np.max(np.abs(a))
+    i, j = np.mgrid[0:nx, 0:ny]
+    dx = 1.2 * np.sin(2 * np.pi * i / 700.0) * np.cos(2 * np.pi * j / 500.0)
+    dy = -0.8 * np.cos(2 * np.pi * i / 300.0)
+    wn = ctx.warp(a, dx, dy)
+    try:
+        ctx.set_warp_method("direct")
+        wd = ctx.warp(a, dx, dy)
+    finally:
+        ctx.set_warp_method("auto")
+    assert np.max(np.abs(wn - wd)) < 1e-10 * scale
