@@ -17,6 +17,7 @@
 #include "interp_cc.hpp"
 #include "whole_kernels.hpp"
 #include "nufft_kernels.hpp"
+#include "fft_rows.hpp"
 #include "track_launch.hpp"
 
 using namespace flct;
@@ -39,6 +40,12 @@ struct flct_hip_ctx {
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   DevBuf f1, f2, vx, vy, vm, list, count, gw, filt, absmax, wtmp, wF, wtw1, wtw2, w3, w4, w5,
       hs1, hs2, mean1, mean2, flags, fcount, segtab, elist, ecount, nuT, nuR, nuTab;
+  // tables of the row transforms (fft_rows.hpp) by (length, sign)
+  struct DftTab {
+    int n, sign, logm, bluestein;
+    DevBuf buf;  // twiddles (M/2) | chirp (n) | chirp transform (M), as double2
+  };
+  std::vector<DftTab> dft_tabs;
   // batch: own scratch, table arena and counters, and the cached graph
   DevBuf bscr[7], barena, bcount;
   hipGraph_t graph = nullptr;
@@ -58,6 +65,7 @@ struct flct_hip_ctx {
   double last_total_ms = 0.;
   double last_main_ms = 0.;    // the tracking kernel alone
   double last_refine_ms = 0.;  // FP64 re-evaluation of flagged pixels (split-BF16 form)
+  int nu_nx = 0, nu_ny = 0;  // image size of the NUFFT tables in nuTab
   int warp_method = FLCT_HIP_WARP_AUTO;
   int last_warp_method = 0;
 };
@@ -139,11 +147,14 @@ extern "C" int flct_hip_destroy(flct_hip_ctx* c) {
   DevBuf* bufs[] = {&c->f1, &c->f2, &c->vx, &c->vy, &c->vm, &c->list, &c->count, &c->gw,
                     &c->filt, &c->absmax, &c->wtmp, &c->wF, &c->wtw1, &c->wtw2, &c->w3,
                     &c->w4, &c->w5, &c->hs1, &c->hs2, &c->mean1, &c->mean2, &c->flags,
-                    &c->fcount, &c->segtab, &c->elist, &c->ecount};
+                    &c->fcount, &c->segtab, &c->elist, &c->ecount, &c->nuT, &c->nuR,
+                    &c->nuTab};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (DevBuf& b : c->bscr)
     if (b.p) (void)hipFree(b.p);
+  for (auto& t : c->dft_tabs)
+    if (t.buf.p) (void)hipFree(t.buf.p);
   if (c->barena.p) (void)hipFree(c->barena.p);
   if (c->bcount.p) (void)hipFree(c->bcount.p);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -554,6 +565,9 @@ static int ensure_scratch(flct_hip_ctx* c, const Plan& pl, Scratch* sc) {
 }
 
 static void root_table(std::vector<double>& t, int n);
+static int row_dft(flct_hip_ctx* c, const double* in, long long irs, long long ies, bool real_in,
+                   int n, int nout, int nrows, double* out, long long ors, long long oes, int sign,
+                   const double* tw_direct);
 
 // sigma = 0 (flct.c:567-586, 666-675, 802-808, 920-947): one cross_cor of
 // the whole nx x ny images with window 1 and whole-image means; threshold and
@@ -592,17 +606,15 @@ static int track_whole(flct_hip_ctx* c, const double* f1, const double* f2, int 
   hipLaunchKernelGGL(k_pack_centered, dim3(nb), dim3(256), 0, s, f1, f2, (const double*)means, n, z);
   HIPCHK(hipEventRecord(c->ev[1], s));
   // forward 2-D transform of z (rows, then columns), e^{-2 pi i}
-  hipLaunchKernelGGL(k_dft<false>, dim3(nb), dim3(256), 0, s, (const double*)z, (long long)ny, 1LL,
-                     ny, ny, nx, t, (long long)ny, 1LL, -1, tw1);
-  hipLaunchKernelGGL(k_dft<false>, dim3(nb), dim3(256), 0, s, (const double*)t, 1LL, (long long)ny,
-                     nx, nx, ny, z, 1LL, (long long)ny, -1, tw2);
+  if ((rc = row_dft(c, z, ny, 1, false, ny, ny, nx, t, ny, 1, -1, tw1)) ||
+      (rc = row_dft(c, t, 1, ny, false, nx, nx, ny, z, 1, ny, -1, tw2)))
+    return rc;
   hipLaunchKernelGGL(k_cross_full, dim3(nb), dim3(256), 0, s, (const double*)z, nx, ny,
                      o->filter ? 1 : 0, o->kr, t);
   // inverse (c2r: real part of the backward transform)
-  hipLaunchKernelGGL(k_dft<false>, dim3(nb), dim3(256), 0, s, (const double*)t, (long long)ny, 1LL,
-                     ny, ny, nx, z, (long long)ny, 1LL, +1, tw1);
-  hipLaunchKernelGGL(k_dft<false>, dim3(nb), dim3(256), 0, s, (const double*)z, 1LL, (long long)ny,
-                     nx, nx, ny, t, 1LL, (long long)ny, +1, tw2);
+  if ((rc = row_dft(c, t, ny, 1, false, ny, ny, nx, z, ny, 1, +1, tw1)) ||
+      (rc = row_dft(c, z, 1, ny, false, nx, nx, ny, t, 1, ny, +1, tw2)))
+    return rc;
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned int*>(red), 4);
   hipLaunchKernelGGL(k_first_max_init, dim3(1), dim3(64), 0, s, red + 1);
   double* sh = z;  // shifted |c|, reusing the spectrum buffer
@@ -997,6 +1009,133 @@ static void root_table(std::vector<double>& t, int n) {
 
 static unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// in-place radix-2 transform on the host, sign -1 (tables of the row DFTs)
+static void host_fft(std::vector<double>& re, std::vector<double>& im, int logm) {
+  const int M = 1 << logm;
+  for (int i = 0; i < M; ++i) {
+    const int j = (int)(__builtin_bitreverse32((unsigned)i) >> (32 - logm));
+    if (j > i) {
+      std::swap(re[i], re[j]);
+      std::swap(im[i], im[j]);
+    }
+  }
+  for (int len = 2; len <= M; len <<= 1) {
+    for (int j = 0; j < len / 2; ++j) {
+      const double a = -2. * M_PI * (double)j / (double)len;
+      const double wr = std::cos(a), wi = std::sin(a);
+      for (int i = 0; i < M; i += len) {
+        const int i0 = i + j, i1 = i0 + len / 2;
+        const double vr = re[i1] * wr - im[i1] * wi, vi = re[i1] * wi + im[i1] * wr;
+        re[i1] = re[i0] - vr;
+        im[i1] = im[i0] - vi;
+        re[i0] += vr;
+        im[i0] += vi;
+      }
+    }
+  }
+}
+
+// tables of the length-n row transform with the given sign (cached)
+static int dft_tables(flct_hip_ctx* c, int n, int sign, const flct_hip_ctx::DftTab** out) {
+  for (const auto& t : c->dft_tabs)
+    if (t.n == n && t.sign == sign) {
+      *out = &t;
+      return FLCT_HIP_OK;
+    }
+  flct_hip_ctx::DftTab t;
+  t.n = n;
+  t.sign = sign;
+  t.bluestein = (n & (n - 1)) != 0;
+  int l = 0;
+  while ((1 << l) < (t.bluestein ? 2 * n - 1 : n)) ++l;
+  t.logm = l;
+  const int M = 1 << l;
+  if (l > kRowDftMaxLog2) return FLCT_HIP_E_UNSUPPORTED;
+  std::vector<double> h;  // double2 entries
+  h.reserve(2 * (M / 2 + n + M));
+  for (int e = 0; e < M / 2; ++e) {
+    h.push_back(std::cos(2. * M_PI * e / M));
+    h.push_back(std::sin(2. * M_PI * e / M));
+  }
+  if (t.bluestein) {
+    // w_m = e^{sign pi i m^2 / n}, the exponent reduced mod 2n exactly
+    std::vector<double> wr(n), wi(n);
+    for (int m = 0; m < n; ++m) {
+      const long long r = ((long long)m * m) % (2LL * n);
+      const double a = (double)sign * M_PI * (double)r / (double)n;
+      wr[m] = std::cos(a);
+      wi[m] = std::sin(a);
+      h.push_back(wr[m]);
+      h.push_back(wi[m]);
+    }
+    std::vector<double> br(M, 0.), bi(M, 0.);
+    for (int m = 0; m < n; ++m) {
+      br[m] = wr[m];
+      bi[m] = -wi[m];
+      if (m) {
+        br[M - m] = wr[m];
+        bi[M - m] = -wi[m];
+      }
+    }
+    host_fft(br, bi, l);
+    for (int q = 0; q < M; ++q) {
+      const int k = (int)(__builtin_bitreverse32((unsigned)q) >> (32 - l));
+      h.push_back(br[k] / M);
+      h.push_back(bi[k] / M);
+    }
+  }
+  int rc;
+  if ((rc = ensure(t.buf, h.size() * 8))) return rc;
+  HIPCHK(hipMemcpy(t.buf.p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  c->dft_tabs.push_back(t);
+  *out = &c->dft_tabs.back();
+  return FLCT_HIP_OK;
+}
+
+// out[r][k] = sum_t in[r][t] e^{sign 2 pi i t k / n}, k < nout, on the
+// context's stream: k_row_dft for n <= 4096, else the direct k_dft with the
+// root table tw_direct (root_table(n))
+static int row_dft(flct_hip_ctx* c, const double* in, long long irs, long long ies, bool real_in,
+                   int n, int nout, int nrows, double* out, long long ors, long long oes, int sign,
+                   const double* tw_direct) {
+  hipStream_t s = c->stream;
+  const flct_hip_ctx::DftTab* t = nullptr;
+  if (n <= (1 << (kRowDftMaxLog2 - 1)) && dft_tables(c, n, sign, &t) == FLCT_HIP_OK) {
+    const int M = 1 << t->logm;
+    const double2* base = (const double2*)t->buf.p;
+    RowDftArgs q;
+    q.in = in;
+    q.irs = irs;
+    q.ies = ies;
+    q.real_in = real_in ? 1 : 0;
+    q.n = n;
+    q.nout = nout;
+    q.nrows = nrows;
+    q.out = out;
+    q.ors = ors;
+    q.oes = oes;
+    q.sign = sign;
+    q.logm = t->logm;
+    q.bluestein = t->bluestein;
+    q.tw = base;
+    q.chirp = base + M / 2;
+    q.bhat = base + M / 2 + n;
+    const size_t lds = (size_t)M * 16;
+    if (lds > c->lds_per_block) return FLCT_HIP_E_UNSUPPORTED;
+    (void)hipFuncSetAttribute((const void*)k_row_dft, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(k_row_dft, dim3(nrows), dim3(kRowDftThreads), lds, s, q);
+  } else if (real_in) {
+    hipLaunchKernelGGL(k_dft<true>, dim3(nblk((long long)nrows * nout, 256)), dim3(256), 0, s, in,
+                       irs, ies, n, nout, nrows, out, ors, oes, sign, tw_direct);
+  } else {
+    hipLaunchKernelGGL(k_dft<false>, dim3(nblk((long long)nrows * nout, 256)), dim3(256), 0, s, in,
+                       irs, ies, n, nout, nrows, out, ors, oes, sign, tw_direct);
+  }
+  HIPCHK(hipGetLastError());
+  return FLCT_HIP_OK;
+}
+
 // Gauss-Legendre nodes and weights on [-1, 1] (Newton on P_n)
 static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
   x.resize(n);
@@ -1056,35 +1195,45 @@ static bool nufft_fits(int nx, int ny) {
   return nx >= 16 && ny >= 16 && log2_grid(nx) <= kNuMaxLog2 && log2_grid(ny) <= kNuMaxLog2;
 }
 
-// warp_frac2d as a type-2 NUFFT on the half spectrum c->wF (already formed)
-static int warp_nufft(flct_hip_ctx* c, const double* delx, const double* dely, double* out, int nx,
-                      int ny, int transp) {
-  hipStream_t s = c->stream;
-  const int lmx = log2_grid(nx), lmy = log2_grid(ny);
-  const int Mx = 1 << lmx, My = 1 << lmy;
+// NUFFT tables of an nx x ny warp (dcx | dcy | twx | twy as doubles), built
+// and uploaded once per image size; synchronous, before any stream work
+static int nufft_prepare(flct_hip_ctx* c, int nx, int ny) {
+  if (c->nu_nx == nx && c->nu_ny == ny && c->nuTab.p) return FLCT_HIP_OK;
+  const int Mx = 1 << log2_grid(nx), My = 1 << log2_grid(ny);
   std::vector<double> dcx, dcy;
   nufft_deconv(nx, Mx, dcx);
   nufft_deconv(ny, My, dcy);
-  // table block: dcx | dcy | twx | twy (doubles)
   std::vector<double> tab;
   tab.reserve(nx + ny + Mx + My);
   tab.insert(tab.end(), dcx.begin(), dcx.end());
   tab.insert(tab.end(), dcy.begin(), dcy.end());
-  const size_t otx = tab.size();
   for (int e = 0; e < Mx / 2; ++e) {
     tab.push_back(std::cos(2. * M_PI * e / Mx));
     tab.push_back(std::sin(2. * M_PI * e / Mx));
   }
-  const size_t oty = tab.size();
   for (int e = 0; e < My / 2; ++e) {
     tab.push_back(std::cos(2. * M_PI * e / My));
     tab.push_back(std::sin(2. * M_PI * e / My));
   }
   int rc;
-  if ((rc = ensure(c->nuTab, tab.size() * 8)) || (rc = ensure(c->nuT, (size_t)nx * My * 16)) ||
-      (rc = ensure(c->nuR, (size_t)My * Mx * 8)))
+  if ((rc = ensure(c->nuTab, tab.size() * 8))) return rc;
+  HIPCHK(hipMemcpy(c->nuTab.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+  c->nu_nx = nx;
+  c->nu_ny = ny;
+  return FLCT_HIP_OK;
+}
+
+// warp_frac2d as a type-2 NUFFT on the half spectrum c->wF (already formed;
+// tables from nufft_prepare)
+static int warp_nufft(flct_hip_ctx* c, const double* delx, const double* dely, double* out, int nx,
+                      int ny, int transp) {
+  hipStream_t s = c->stream;
+  const int lmx = log2_grid(nx), lmy = log2_grid(ny);
+  const int Mx = 1 << lmx, My = 1 << lmy;
+  const size_t otx = (size_t)nx + ny, oty = otx + Mx;
+  int rc;
+  if ((rc = ensure(c->nuT, (size_t)nx * My * 16)) || (rc = ensure(c->nuR, (size_t)My * Mx * 8)))
     return rc;
-  HIPCHK(hipMemcpyAsync(c->nuTab.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, s));
   const double* tb = (const double*)c->nuTab.p;
   NufftArgs q;
   q.F = (const double*)c->wF.p;
@@ -1140,21 +1289,30 @@ extern "C" int flct_hip_warp_device(flct_hip_ctx* c, const double* arr, const do
     return rc;
   HIPCHK(hipMemcpyAsync(c->wtw1.p, t1.data(), t1.size() * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(c->wtw2.p, t2.data(), t2.size() * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(hipEventRecord(c->ev[0], s));
-  // rows: Frow[i][b] = sum_j arr[i][j] e(-j b / ny), b = 0..ny/2
-  hipLaunchKernelGGL(k_dft<true>, dim3(nblk((long long)nx * nb, 256)), dim3(256), 0, s, arr,
-                     (long long)ny, 1LL, ny, nb, nx, (double*)c->wtmp.p, (long long)nb, 1LL, -1,
-                     (const double*)c->wtw1.p);
-  // columns: F[a][b] = sum_i Frow[i][b] e(-i a / nx)
-  hipLaunchKernelGGL(k_dft<false>, dim3(nblk((long long)nb * nx, 256)), dim3(256), 0, s,
-                     (const double*)c->wtmp.p, 1LL, (long long)nb, nx, nx, nb, (double*)c->wF.p,
-                     1LL, (long long)nb, -1, (const double*)c->wtw2.p);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c->ev[1], s));
   const bool nufft = (c->warp_method == FLCT_HIP_WARP_NUFFT) ||
                      (c->warp_method == FLCT_HIP_WARP_AUTO && nufft_fits(nx, ny));
   if (nufft) {
     if (!nufft_fits(nx, ny)) return FLCT_HIP_E_UNSUPPORTED;
+    if ((rc = nufft_prepare(c, nx, ny))) return rc;
+  }
+  // row-transform tables (cached), before the timed region
+  {
+    const flct_hip_ctx::DftTab* t;
+    if (ny <= (1 << (kRowDftMaxLog2 - 1))) (void)dft_tables(c, ny, -1, &t);
+    if (nx <= (1 << (kRowDftMaxLog2 - 1))) (void)dft_tables(c, nx, -1, &t);
+  }
+  HIPCHK(hipEventRecord(c->ev[0], s));
+  // rows: Frow[i][b] = sum_j arr[i][j] e(-j b / ny), b = 0..ny/2
+  if ((rc = row_dft(c, arr, ny, 1, true, ny, nb, nx, (double*)c->wtmp.p, nb, 1, -1,
+                    (const double*)c->wtw1.p)))
+    return rc;
+  // columns: F[a][b] = sum_i Frow[i][b] e(-i a / nx)
+  if ((rc = row_dft(c, (const double*)c->wtmp.p, 1, nb, false, nx, nx, nb, (double*)c->wF.p, 1, nb,
+                    -1, (const double*)c->wtw2.p)))
+    return rc;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->ev[1], s));
+  if (nufft) {
     if ((rc = warp_nufft(c, delx, dely, out, nx, ny, transp))) return rc;
     HIPCHK(hipEventRecord(c->ev[2], s));
     HIPCHK(hipStreamSynchronize(s));
@@ -1226,11 +1384,10 @@ extern "C" int flct_hip_shift(flct_hip_ctx* c, const double* arr, double delx, d
   double* A = (double*)c->w3.p;
   double* B = (double*)c->w4.p;
   // forward 2D transform (warp.c:806)
-  hipLaunchKernelGGL(k_dft<true>, dim3(nblk(n, 256)), dim3(256), 0, s, (const double*)c->f1.p,
-                     (long long)ny, 1LL, ny, ny, nx, A, (long long)ny, 1LL, -1,
-                     (const double*)c->wtw1.p);
-  hipLaunchKernelGGL(k_dft<false>, dim3(nblk(n, 256)), dim3(256), 0, s, (const double*)A, 1LL,
-                     (long long)ny, nx, nx, ny, B, 1LL, (long long)ny, -1, (const double*)c->wtw2.p);
+  if ((rc = row_dft(c, (const double*)c->f1.p, ny, 1, true, ny, ny, nx, A, ny, 1, -1,
+                    (const double*)c->wtw1.p)) ||
+      (rc = row_dft(c, A, 1, ny, false, nx, nx, ny, B, 1, ny, -1, (const double*)c->wtw2.p)))
+    return rc;
   // phase ramp (warp.c:811-852)
   double dxarg, dyarg;
   if (transp) {
@@ -1242,11 +1399,9 @@ extern "C" int flct_hip_shift(flct_hip_ctx* c, const double* arr, double delx, d
   }
   hipLaunchKernelGGL(k_phase_ramp, dim3(nblk(n, 256)), dim3(256), 0, s, B, nx, ny, dxarg, dyarg);
   // backward 2D transform (warp.c:855)
-  hipLaunchKernelGGL(k_dft<false>, dim3(nblk(n, 256)), dim3(256), 0, s, (const double*)B,
-                     (long long)ny, 1LL, ny, ny, nx, A, (long long)ny, 1LL, +1,
-                     (const double*)c->wtw1.p);
-  hipLaunchKernelGGL(k_dft<false>, dim3(nblk(n, 256)), dim3(256), 0, s, (const double*)A, 1LL,
-                     (long long)ny, nx, nx, ny, B, 1LL, (long long)ny, +1, (const double*)c->wtw2.p);
+  if ((rc = row_dft(c, B, ny, 1, false, ny, ny, nx, A, ny, 1, +1, (const double*)c->wtw1.p)) ||
+      (rc = row_dft(c, A, 1, ny, false, nx, nx, ny, B, 1, ny, +1, (const double*)c->wtw2.p)))
+    return rc;
   hipLaunchKernelGGL(k_real_scale, dim3(nblk(n, 256)), dim3(256), 0, s, (const double*)B,
                      (double*)c->vx.p, n, (1. / ((double)nx * ny)));
   HIPCHK(hipGetLastError());
