@@ -202,6 +202,18 @@ def main():
         roof["executed_mfma_flops_per_px"] = round(ex_flops / hr["tracked"])
         roof["mfma_util"] = round(ex_flops / (hr["main_ms"] * 1e-3) / 1e12 / peak, 4)
 
+    # warp_frac2d of image 1 by the FP64 velocities (the NUFFT evaluation)
+    ctx.upload(d["vx"], ref[0])
+    ctx.upload(d["vy"], ref[1])
+    wms = []
+    for _ in range(1 + max(1, args.steps)):
+        ctx.warp_device(d["f1"], d["vx"], d["vy"], d["vm"], nx, ny)
+        wms.append(ctx.last_stats()["total_ms"])
+    warp = {"workload": f"warp_frac2d of the {nx}x{ny} image 1 by the FP64 velocities",
+            "method": ctx.last_warp_method(), "ms": round(float(np.mean(wms[1:])), 3),
+            "px_per_s": round(nx * ny / (np.mean(wms[1:]) * 1e-3), 1)}
+    log(f"bench: warp {nx}x{ny}: {warp['ms']:.2f} ms ({warp['method']})")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle baseline")
@@ -234,6 +246,7 @@ def main():
         "fp64_vs_oracle_max_err_px": parity,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "warp": warp,
     }
     for k in d.values():
         ctx.free(k)
@@ -284,6 +297,25 @@ def uneven_batch(args):
         res[form] = {"ms": float(np.mean(ms)), "px_per_s": tracked / (np.mean(ms) * 1e-3),
                      "tracked": tracked, "refined": sum(x["refined"] for x in st)}
         log(f"bench: {form}: {np.mean(ms):.1f} ms per batch, {res[form]['px_per_s'] / 1e6:.1f} Mpx/s")
+    # the warp of every image 1 by its split-BF16 velocities, timed (device
+    # time per call, summed over the pairs), then the same for one 2048^2
+    # pair with the reference's direct per-pixel sum
+    wms = 0.0
+    for n, sigma, d in pairs:
+        ctx.warp_device(d["f1"], d["split_bf16_vx"], d["split_bf16_vy"], d["dx"], n, n)
+        wms += ctx.last_stats()["total_ms"]
+    wmethod = ctx.last_warp_method()
+    direct = None
+    big = [p_ for p_ in pairs if p_[0] == 2048]
+    if big:
+        n, sigma, d = big[0]
+        ctx.set_warp_method("direct")
+        try:
+            ctx.warp_device(d["f1"], d["split_bf16_vx"], d["split_bf16_vy"], d["dy"], n, n)
+            direct = round(ctx.last_stats()["total_ms"], 3)
+        finally:
+            ctx.set_warp_method("auto")
+    log(f"bench: warps of all pairs {wms:.1f} ms ({wmethod}); one 2048^2 direct sum {direct} ms")
     # errors against FP64 and the warp residual of the split-BF16 velocities
     sq = {"fp32": 0.0, "split_bf16": 0.0}
     mx = {"fp32": 0.0, "split_bf16": 0.0}
@@ -323,6 +355,10 @@ def uneven_batch(args):
         "max_vel_err_vs_fp64": mx,
         "warp_residual_rms": {"split_bf16_velocities": float(np.sqrt(np.mean(np.square(rres)))),
                               "unwarped": float(np.sqrt(np.mean(np.square(rbase))))},
+        "warp": {"method": wmethod, "ms_all_pairs": round(wms, 3),
+                 "px_per_s": round(total_px / (wms * 1e-3), 1),
+                 "direct_sum_ms_one_2048_pair": direct},
+        "tracking_plus_warp_ms": round(hr["ms"] + wms, 3),
     }
     print(json.dumps(out))
 
