@@ -342,3 +342,36 @@ def test_sigma0_subpixel_shift_and_negative_sigma():
     assert abs(vx2[0, 0] + vx[0, 0]) < 1e-6 and abs(vy2[0, 0] + vy[0, 0]) < 1e-6
     with pytest.raises(RuntimeError):
         O.flct(g, h, 1.0, 1.0, -4.0)
+
+
+def test_warp_fixture_was_made_in_double_precision(golden):
+    """The reference's warp fixture (hashgauss.dat f2 = shift_frac2d(f1,
+    deltax, deltay), README.md:250-262) matches warp_frac2d in FP64 to a few
+    float32 steps of the stored file, while a strict single-precision
+    evaluation of shift_frac2d.pro's non-uniform branch (float phases with
+    the float pi, float products and sums, IDL-io-procedures/
+    shift_frac2d.pro:200-251) misses it by ~1e-5 on the sampled pixels: the
+    fixture pins the FP64 form, and an FP32 warp form would move away from
+    it.  Sampled pixels (numpy float32 restatement)."""
+    f1, f2, dx, dy = golden["f1"], golden["f2"], golden["dx"], golden["dy"]
+    nx, ny = f1.shape
+    rng = np.random.default_rng(0)
+    pix = rng.choice(nx * ny, 200, replace=False)
+    # strict float32 restatement, IDL layout: x = C column j, y = C row i
+    ff = (np.fft.fft2(f1.T) / (nx * ny)).astype(np.complex64)  # [kx_idl, ky_idl]
+    kx = np.asarray(O.make_freq(ny), np.float32)
+    ky = np.asarray(O.make_freq(nx), np.float32)
+    pi2 = np.float32(np.float32(np.pi) * np.float32(2))
+    single = np.empty(pix.size)
+    for q, p in enumerate(pix):
+        i, j = divmod(int(p), ny)  # C row i = IDL y, C column j = IDL x
+        xarg = pi2 * (np.float32(j) - np.float32(dx[i, j])) / np.float32(ny)
+        yarg = pi2 * (np.float32(i) - np.float32(dy[i, j])) / np.float32(nx)
+        ax, ay = kx * xarg, ky * yarg
+        ex = (np.cos(ax) + 1j * np.sin(ax)).astype(np.complex64)
+        ey = (np.cos(ay) + 1j * np.sin(ay)).astype(np.complex64)
+        single[q] = np.sum((np.outer(ex, ey) * ff).astype(np.complex64), dtype=np.complex64).real
+    wo = O.warp_frac2d(f1, dx, dy).ravel()[pix]
+    ref = f2.ravel()[pix]
+    assert np.max(np.abs(wo - ref)) < 5e-7
+    assert np.max(np.abs(single - ref)) > 5e-6
