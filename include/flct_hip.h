@@ -33,7 +33,8 @@ extern "C" {
 
 /* arithmetic forms of the tracker */
 #define FLCT_HIP_FP64 0        /* plain form: FP64 throughout (drop-in) */
-#define FLCT_HIP_FP32 1        /* FP32 transforms, FP64 means, fit and outputs */
+#define FLCT_HIP_FP32 1        /* FP32 transforms, FP64 means, fit and outputs;
+                                  unsettled pixels re-evaluated in FP64 */
 #define FLCT_HIP_SPLIT_BF16 2  /* DFTs as split-BF16 MFMA contractions, FP32 accumulate */
 
 typedef struct flct_hip_ctx flct_hip_ctx;

@@ -60,11 +60,11 @@ struct flct_hip_ctx {
   int force_path = -1;    // path under test while tuning
   int last_path = 0;
   long long last_tracked = 0;
-  long long refined = 0;  // pixels re-evaluated in FP64 by the last split-BF16 call
+  long long refined = 0;  // pixels re-evaluated in FP64 by the last split-BF16 or FP32 call
   double last_kernel_ms = 0.;
   double last_total_ms = 0.;
   double last_main_ms = 0.;    // the tracking kernel alone
-  double last_refine_ms = 0.;  // FP64 re-evaluation of flagged pixels (split-BF16 form)
+  double last_refine_ms = 0.;  // FP64 re-evaluation of flagged pixels (split-BF16, FP32)
   int nu_nx = 0, nu_ny = 0;  // image size of the NUFFT tables in nuTab
   int warp_method = FLCT_HIP_WARP_AUTO;
   int last_warp_method = 0;
@@ -388,7 +388,7 @@ struct Scratch {
   double* hs2 = nullptr;
   double* mean1 = nullptr;
   double* mean2 = nullptr;
-  int* flags = nullptr;  // split-BF16 pixels for FP64 re-evaluation
+  int* flags = nullptr;  // split-BF16 and FP32 pixels for FP64 re-evaluation
   int* elist = nullptr;  // edge-column pixels of the row-segment path
 };
 
@@ -399,7 +399,7 @@ void scratch_sizes(const Plan& pl, size_t sz[7]) {
   sz[0] = pl.sparse ? npix * sizeof(int) : 0;
   sz[1] = sz[2] = (size_t)pl.nx * nsx * 8;
   sz[3] = sz[4] = (size_t)nrx * nsx * 8;
-  sz[5] = (pl.path == PATH_BF16_SEG || pl.path == PATH_BF16_PIXEL) ? npix * sizeof(int) : 0;
+  sz[5] = (pl.path != PATH_FP64) ? npix * sizeof(int) : 0;
   sz[6] = pl.seg ? (pl.sparse ? npix : (long long)pl.nx * (N - 1)) * sizeof(int) : 0;
 }
 
@@ -474,8 +474,11 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
   ta.vy = vy;
   ta.vm = vm;
   ta.absmax = bf16 ? absmax : nullptr;
-  ta.flag_list = bf16 ? sc.flags : nullptr;
-  ta.flag_count = bf16 ? counts + 1 : nullptr;
+  // the split-BF16 and FP32 kernels list the pixels their surface cannot
+  // settle; those are tracked again by the FP64 kernels
+  const bool refine = bf16 || pl.path == PATH_FP32;
+  ta.flag_list = refine ? sc.flags : nullptr;
+  ta.flag_count = refine ? counts + 1 : nullptr;
   if (ev) HIPCHK(hipEventRecord(ev[1], s));
   int rc = FLCT_HIP_OK;
   if (bf16) {
@@ -497,6 +500,11 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
     } else if ((rc = dispatch_track_bf16(N, lc, ta))) {
       return rc;
     }
+  } else if (pl.path == PATH_FP32) {
+    if ((rc = (M <= 12) ? dispatch_track_f32_lo(M, lc, ta) : dispatch_track_f32_hi(M, lc, ta)))
+      return rc;
+  }
+  if (refine) {
     if (ev) HIPCHK(hipEventRecord(ev[4], s));
     // FP64 re-evaluation of the flagged pixels (count read on the device)
     TrackArgs tr = ta;
@@ -507,8 +515,6 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
     tr.flag_list = nullptr;
     tr.flag_count = nullptr;
     rc = (M <= 12) ? dispatch_track_f64_lo(M, lc, tr) : dispatch_track_f64_hi(M, lc, tr);
-  } else if (pl.path == PATH_FP32) {
-    rc = (M <= 12) ? dispatch_track_f32_lo(M, lc, ta) : dispatch_track_f32_hi(M, lc, ta);
   } else {
     if (ta.hires)
       rc = (M <= 12) ? dispatch_track_f64h_lo(M, lc, ta) : dispatch_track_f64h_hi(M, lc, ta);
@@ -758,13 +764,13 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
   unsigned int hc[4];
   HIPCHK(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  const bool bf16 = (pl.path == PATH_BF16_SEG || pl.path == PATH_BF16_PIXEL);
+  const bool refine = (pl.path != PATH_FP64);
   c->last_path = pl.path;
   c->last_tracked = pl.sparse ? (long long)hc[0] : (long long)nx * ny;
-  c->refined = bf16 ? (long long)hc[1] : 0;
+  c->refined = refine ? (long long)hc[1] : 0;
   c->last_kernel_ms = ev_ms(c->ev[1], c->ev[2]);
   c->last_total_ms = ev_ms(c->ev[0], c->ev[3]);
-  if (bf16) {
+  if (refine) {
     c->last_main_ms = ev_ms(c->ev[1], c->ev[4]);
     c->last_refine_ms = ev_ms(c->ev[4], c->ev[2]);
   } else {
@@ -984,8 +990,7 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
   for (int k = 0; k < npairs; ++k) {
     const unsigned int* ck = hc.data() + (size_t)k * 8;
     pairs[k].tracked = pl[k].sparse ? (long long)ck[0] : (long long)pl[k].nx * pl[k].ny;
-    pairs[k].refined =
-        (pl[k].path == PATH_BF16_SEG || pl[k].path == PATH_BF16_PIXEL) ? (long long)ck[1] : 0;
+    pairs[k].refined = (pl[k].path != PATH_FP64) ? (long long)ck[1] : 0;
     tracked += pairs[k].tracked;
     refined += pairs[k].refined;
   }

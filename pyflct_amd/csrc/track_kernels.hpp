@@ -57,6 +57,14 @@ struct TrackArgs {
   double* vm;
 };
 
+// The FP32 form lists the pixels its surface cannot settle for re-evaluation
+// in FP64, like the split-BF16 form: another cell within kTieTol32 of the
+// maximum, or a Taylor fit whose first-order response to relative errors
+// kRelErr32 of the nine values exceeds kFitTol32 pixels.
+constexpr double kTieTol32 = 1e-5;
+constexpr double kRelErr32 = 1e-6;
+constexpr double kFitTol32 = 1e-2;
+
 // slots per block for box size 4*M, real type T
 template <int M, class T>
 struct TrackCfg {
@@ -234,6 +242,7 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
       f2bar = a.mean2[(long long)imin * ns + jmin];
     }
     double shiftx = 0., shifty = 0.;
+    bool flagged = false;  // FP32: re-evaluate in FP64 (read by thread stid == 0 only)
     for (int pass = 0; pass < npass; ++pass) {
       // ------------- windows: separable Gaussian centred on the pixel, or
       // moved by -d/2 (image 1) and +d/2 (image 2) in the bias pass
@@ -347,6 +356,10 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
       }
       __syncthreads();
       // ------------- phase 2: row transforms, cross power, inverse along jj
+      // (FP32: gmx[0], read in phase 1, now counts near-maximum cells)
+      if constexpr (sizeof(T) == 4) {
+        if (active && stid == 0) gmx[0] = 0ull;
+      }
       if (active) {
         const int l = fresh(lane), quad = fresh(qd);
         const int t = quad;
@@ -473,6 +486,17 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
         best = min(best, (unsigned)__builtin_amdgcn_mov_dpp((int)best, kQuadXor1, 0xF, 0xF, true));
         best = min(best, (unsigned)__builtin_amdgcn_mov_dpp((int)best, kQuadXor2, 0xF, 0xF, true));
         if (l == 0 && best != 0xffffffffu) atomicMin(aidx, best);
+        if constexpr (sizeof(T) == 4) {
+          if (a.flag_list && gmax != 0.) {
+            const double thr = gmax * (1. - kTieTol32);
+            unsigned cnt = 0;
+            static_for<M>([&](auto S) {
+              cnt += ((double)va[S] >= thr) ? 1u : 0u;
+              cnt += ((double)vb[S] >= thr) ? 1u : 0u;
+            });
+            if (cnt) atomicAdd(reinterpret_cast<unsigned int*>(gmx), cnt);
+          }
+        }
       }
       __syncthreads();
       const unsigned loc = *aidx;
@@ -567,6 +591,16 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
           const double fxy = 0.25 * (nb[8] + nb[0] - nb[6] - nb[2]);
           shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
           shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
+          if constexpr (sizeof(T) == 4) {
+            const double det = fxx * fyy - fxy * fxy;
+            const double hn = fabs(fxx) + fabs(fyy) + 2. * fabs(fxy);
+            const double bound =
+                kRelErr32 * gmax * hn * (1. + 4. * (fabs(shiftsubx) + fabs(shiftsuby)));
+            if (!(bound <= kFitTol32 * fabs(det))) flagged = true;
+          }
+        }
+        if constexpr (sizeof(T) == 4) {
+          if (nonzero && *reinterpret_cast<const unsigned int*>(gmx) > 1u) flagged = true;
         }
         if (hq < 0) {
           res[0] = ((double)ixmax + shiftsubx) - (double)(N / 2);
@@ -582,6 +616,9 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
       a.vx[p] = shifty * a.deltinv * a.deltas;
       a.vy[p] = shiftx * a.deltinv * a.deltas;
       a.vm[p] = 1.;
+      if constexpr (sizeof(T) == 4) {
+        if (flagged && a.flag_list) a.flag_list[atomicAdd(a.flag_count, 1u)] = (int)p;
+      }
     }
   }
 }
