@@ -215,6 +215,11 @@ int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, do
   long long grid = c.num_cu;
   if (grid > nseg) grid = nseg;
   if (grid < 1) return FLCT_HIP_OK;
+  // XCD order when the bands split evenly over the 8 XCDs and each XCD's
+  // blocks evenly over its bands (k_track_seg)
+  s.xchunk = 0;
+  if (grid % 8 == 0 && s.nband % 8 == 0 && (grid / 8) % (s.nband / 8) == 0)
+    s.xchunk = (int)((grid / 8) / (s.nband / 8));
   if (N <= 32)
     hipLaunchKernelGGL(k_track_seg<1>, dim3((unsigned)grid), dim3(C::kThreads), lds, c.stream, s);
   else

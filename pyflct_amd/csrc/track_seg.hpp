@@ -61,6 +61,7 @@ struct SegArgs {
   int width;             // interior columns, ny - N + 1
   int nband;             // column bands of at most 64 pixels
   int nseg;              // nband * nx segments
+  int xchunk;            // > 0: XCD-grouped order with this many row chunks (see k_track_seg)
 };
 
 struct SegCfg {
@@ -168,9 +169,26 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
   }
 
   const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+  // Segment order.  Default: block b takes segments [b nseg / G, (b+1) nseg
+  // / G) of the band-major order, consecutive rows of one band.  XCD order
+  // (xchunk > 0): blocks are dealt to the 8 XCDs round robin, XCD x = b % 8
+  // owns the bands [x nband / 8, (x+1) nband / 8), and its blocks take one
+  // band and one of xchunk row chunks each, adjacent bands side by side; the
+  // blocks of an XCD then read overlapping box columns of the same rows, and
+  // their working set stays in the XCD's L2.
   const int G = gridDim.x;
-  const int s0 = (int)((long long)blockIdx.x * A.nseg / G);
-  const int s1 = (int)((long long)(blockIdx.x + 1) * A.nseg / G);
+  int s0, s1, sband = 0;
+  if (A.xchunk > 0) {
+    const int x = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const int nbg = A.nband >> 3;
+    sband = x * nbg + k % nbg;
+    const int ch = k / nbg;
+    s0 = (int)((long long)ch * A.t.nx / A.xchunk);
+    s1 = (int)((long long)(ch + 1) * A.t.nx / A.xchunk);
+  } else {
+    s0 = (int)((long long)blockIdx.x * A.nseg / G);
+    s1 = (int)((long long)(blockIdx.x + 1) * A.nseg / G);
+  }
 
   for (int sidx = s0; sidx < s1; ++sidx) {
     SegArgsPtr As = seg_args();
@@ -187,8 +205,8 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         if (m > 0. && m < 1e300) sc[X] = ldexp(1., -ilogb(m));
       });
     }
-    const int band = sidx / nx;
-    const int i = sidx - band * nx;
+    const int band = (A.xchunk > 0) ? sband : sidx / nx;
+    const int i = (A.xchunk > 0) ? sidx : sidx - band * nx;
     const int jb0 = A.jlo + (int)((long long)band * A.width / A.nband);
     const int jb1 = A.jlo + (int)((long long)(band + 1) * A.width / A.nband);
     const int P = jb1 - jb0;
