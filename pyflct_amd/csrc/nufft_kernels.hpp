@@ -151,7 +151,11 @@ __global__ void __launch_bounds__(256) k_nufft_gather(NufftArgs q) {
   const double yarg = 2. * kWarpPi * (((double)j) - shifty) * nyinv;
   const int Mx = 1 << q.lmx, My = 1 << q.lmy;
   const double inv2pi = 0.15915494309189533576888376337251;  // 1 / (2 pi)
-  const double tx = xarg * ((double)Mx * inv2pi), ty = yarg * ((double)My * inv2pi);
+  double tx = xarg * ((double)Mx * inv2pi), ty = yarg * ((double)My * inv2pi);
+  // the grid is periodic: bring far displacements back to one period so the
+  // integer tap origin below stays small
+  tx -= (double)Mx * floor(tx * (1. / (double)Mx));
+  ty -= (double)My * floor(ty * (1. / (double)My));
   const double fx = floor(tx - 0.5 * kNuW), fy = floor(ty - 0.5 * kNuW);
   const int mx0 = (int)fx + 1, my0 = (int)fy + 1;
   const double zx0 = tx - (fx + 1.), zy0 = ty - (fy + 1.);

@@ -310,7 +310,9 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
           }
         }
       } else {
-        double ga[2][M], gb[2][M];
+        // two passes over the box: the maxima first, then the scaled values
+        // (recomputed with the same operations, so they are the values the
+        // maxima were taken over; nothing is held across the barrier)
         if (active) {
           const int l = fresh(lane), quad = fresh(qd);
           const double* p1 = a.f1 + (long long)imin * ny + jmin;
@@ -322,10 +324,8 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
             static_for<M>([&](auto Rr) {
               const int ii = 4 * Rr + l;
               const long long off = (long long)ii * ny + c;
-              ga[Tc][Rr] = (wx1[ii] * w1c) * (p1[off] - f1bar);
-              gb[Tc][Rr] = (wx2[ii] * w2c) * (p2[off] - f2bar);
-              m1 = fmax(m1, fabs(ga[Tc][Rr]));
-              m2 = fmax(m2, fabs(gb[Tc][Rr]));
+              m1 = fmax(m1, fabs((wx1[ii] * w1c) * (p1[off] - f1bar)));
+              m2 = fmax(m2, fabs((wx2[ii] * w2c) * (p2[off] - f2bar)));
             });
           });
           m1 = fmax(m1, dpp<kQuadXor1>(m1));
@@ -340,18 +340,28 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
         __syncthreads();
         if (active) {
           const int l = fresh(lane), quad = fresh(qd);
+          const double* p1 = a.f1 + (long long)imin * ny + jmin;
+          const double* p2 = a.f2 + (long long)imin * ny + jmin;
           const double g1max = __longlong_as_double((long long)gmx[0]);
           const double g2max = __longlong_as_double((long long)gmx[1]);
           const double s1 = (g1max > 0.) ? ldexp(1.0, -ilogb(g1max)) : 1.0;
           const double s2 = (g2max > 0.) ? ldexp(1.0, -ilogb(g2max)) : 1.0;
           const int kb = M * brev2(l);
-          static_for<2>([&](auto Tc) {
+#pragma unroll 1
+          for (int Tc = 0; Tc < 2; ++Tc) {
             const int c = quad + Tc * (N / 2);
+            const double w1c = wy1[c], w2c = wy2[c];
             cplx<T> z[M];
-            static_for<M>([&](auto Rr) { z[Rr] = {T(ga[Tc][Rr] * s1), T(gb[Tc][Rr] * s2)}; });
+            static_for<M>([&](auto Rr) {
+              const int ii = 4 * Rr + l;
+              const long long off = (long long)ii * ny + c;
+              const double g1 = (wx1[ii] * w1c) * (p1[off] - f1bar);
+              const double g2 = (wx2[ii] * w2c) * (p2[off] - f2bar);
+              z[Rr] = {T(g1 * s1), T(g2 * s2)};
+            });
             quad_dit<M, -1>(z, l);
             static_for<M>([&](auto S) { Z[zidx<M>(kb + S, c)] = z[S]; });
-          });
+          }
         }
       }
       __syncthreads();

@@ -133,3 +133,26 @@ def test_warp_nufft_large_fields(ctx):
     finally:
         ctx.set_warp_method("auto")
     assert np.max(np.abs(wn - wd)) < 1e-10 * scale
+
+
+def test_warp_nufft_far_shifts(ctx):
+    """Displacements of thousands of pixels: the NUFFT takes the reference's
+    phase arguments (its pi literal makes them not exactly periodic in the
+    shift) and reduces the grid position to one period; it agrees with the
+    direct sum, and with the near shift to the size of the pi literal's
+    phase drift."""
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal((48, 40))
+    dx = rng.standard_normal(a.shape) * 0.7 + 40 * 1000.0
+    dy = rng.standard_normal(a.shape) * 0.7 - 48 * 300.0
+    far = ctx.warp(a, dx, dy)
+    assert ctx.last_warp_method() == "nufft"
+    try:
+        ctx.set_warp_method("direct")
+        far_d = ctx.warp(a, dx, dy)
+    finally:
+        ctx.set_warp_method("auto")
+    scale = np.max(np.abs(a))
+    assert np.max(np.abs(far - far_d)) < 1e-8 * scale
+    near = ctx.warp(a, dx - 40 * 1000.0, dy + 48 * 300.0)
+    assert np.max(np.abs(far - near)) < 1e-6 * scale
