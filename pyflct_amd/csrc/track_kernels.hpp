@@ -310,57 +310,105 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
           }
         }
       } else {
-        // two passes over the box: the maxima first, then the scaled values
-        // (recomputed with the same operations, so they are the values the
-        // maxima were taken over; nothing is held across the barrier)
-        if (active) {
-          const int l = fresh(lane), quad = fresh(qd);
-          const double* p1 = a.f1 + (long long)imin * ny + jmin;
-          const double* p2 = a.f2 + (long long)imin * ny + jmin;
-          double m1 = 0., m2 = 0.;
-          static_for<2>([&](auto Tc) {
-            const int c = quad + Tc * (N / 2);
-            const double w1c = wy1[c], w2c = wy2[c];
-            static_for<M>([&](auto Rr) {
-              const int ii = 4 * Rr + l;
-              const long long off = (long long)ii * ny + c;
-              m1 = fmax(m1, fabs((wx1[ii] * w1c) * (p1[off] - f1bar)));
-              m2 = fmax(m2, fabs((wx2[ii] * w2c) * (p2[off] - f2bar)));
+        // Boxes 44..68: two passes over the box, the maxima first, then the
+        // scaled values recomputed with the same operations (nothing held
+        // across the barrier: one more wave per SIMD).  Otherwise the values
+        // stay in registers between the passes (no second read of the box;
+        // from 72 up the kernel is at one wave per SIMD either way).
+        if constexpr (M >= 11 && M <= 17) {
+          if (active) {
+            const int l = fresh(lane), quad = fresh(qd);
+            const double* p1 = a.f1 + (long long)imin * ny + jmin;
+            const double* p2 = a.f2 + (long long)imin * ny + jmin;
+            double m1 = 0., m2 = 0.;
+            static_for<2>([&](auto Tc) {
+              const int c = quad + Tc * (N / 2);
+              const double w1c = wy1[c], w2c = wy2[c];
+              static_for<M>([&](auto Rr) {
+                const int ii = 4 * Rr + l;
+                const long long off = (long long)ii * ny + c;
+                m1 = fmax(m1, fabs((wx1[ii] * w1c) * (p1[off] - f1bar)));
+                m2 = fmax(m2, fabs((wx2[ii] * w2c) * (p2[off] - f2bar)));
+              });
             });
-          });
-          m1 = fmax(m1, dpp<kQuadXor1>(m1));
-          m2 = fmax(m2, dpp<kQuadXor1>(m2));
-          m1 = fmax(m1, dpp<kQuadXor2>(m1));
-          m2 = fmax(m2, dpp<kQuadXor2>(m2));
-          if (l == 0) {
-            atomicMax(&gmx[0], dbits(m1));
-            atomicMax(&gmx[1], dbits(m2));
+            m1 = fmax(m1, dpp<kQuadXor1>(m1));
+            m2 = fmax(m2, dpp<kQuadXor1>(m2));
+            m1 = fmax(m1, dpp<kQuadXor2>(m1));
+            m2 = fmax(m2, dpp<kQuadXor2>(m2));
+            if (l == 0) {
+              atomicMax(&gmx[0], dbits(m1));
+              atomicMax(&gmx[1], dbits(m2));
+            }
           }
-        }
-        __syncthreads();
-        if (active) {
-          const int l = fresh(lane), quad = fresh(qd);
-          const double* p1 = a.f1 + (long long)imin * ny + jmin;
-          const double* p2 = a.f2 + (long long)imin * ny + jmin;
-          const double g1max = __longlong_as_double((long long)gmx[0]);
-          const double g2max = __longlong_as_double((long long)gmx[1]);
-          const double s1 = (g1max > 0.) ? ldexp(1.0, -ilogb(g1max)) : 1.0;
-          const double s2 = (g2max > 0.) ? ldexp(1.0, -ilogb(g2max)) : 1.0;
-          const int kb = M * brev2(l);
+          __syncthreads();
+          if (active) {
+            const int l = fresh(lane), quad = fresh(qd);
+            const double* p1 = a.f1 + (long long)imin * ny + jmin;
+            const double* p2 = a.f2 + (long long)imin * ny + jmin;
+            const double g1max = __longlong_as_double((long long)gmx[0]);
+            const double g2max = __longlong_as_double((long long)gmx[1]);
+            const double s1 = (g1max > 0.) ? ldexp(1.0, -ilogb(g1max)) : 1.0;
+            const double s2 = (g2max > 0.) ? ldexp(1.0, -ilogb(g2max)) : 1.0;
+            const int kb = M * brev2(l);
 #pragma unroll 1
-          for (int Tc = 0; Tc < 2; ++Tc) {
-            const int c = quad + Tc * (N / 2);
-            const double w1c = wy1[c], w2c = wy2[c];
-            cplx<T> z[M];
-            static_for<M>([&](auto Rr) {
-              const int ii = 4 * Rr + l;
-              const long long off = (long long)ii * ny + c;
-              const double g1 = (wx1[ii] * w1c) * (p1[off] - f1bar);
-              const double g2 = (wx2[ii] * w2c) * (p2[off] - f2bar);
-              z[Rr] = {T(g1 * s1), T(g2 * s2)};
+            for (int Tc = 0; Tc < 2; ++Tc) {
+              const int c = quad + Tc * (N / 2);
+              const double w1c = wy1[c], w2c = wy2[c];
+              cplx<T> z[M];
+              static_for<M>([&](auto Rr) {
+                const int ii = 4 * Rr + l;
+                const long long off = (long long)ii * ny + c;
+                const double g1 = (wx1[ii] * w1c) * (p1[off] - f1bar);
+                const double g2 = (wx2[ii] * w2c) * (p2[off] - f2bar);
+                z[Rr] = {T(g1 * s1), T(g2 * s2)};
+              });
+              quad_dit<M, -1>(z, l);
+              static_for<M>([&](auto S) { Z[zidx<M>(kb + S, c)] = z[S]; });
+            }
+          }
+        } else {
+          double ga[2][M], gb[2][M];
+          if (active) {
+            const int l = fresh(lane), quad = fresh(qd);
+            const double* p1 = a.f1 + (long long)imin * ny + jmin;
+            const double* p2 = a.f2 + (long long)imin * ny + jmin;
+            double m1 = 0., m2 = 0.;
+            static_for<2>([&](auto Tc) {
+              const int c = quad + Tc * (N / 2);
+              const double w1c = wy1[c], w2c = wy2[c];
+              static_for<M>([&](auto Rr) {
+                const int ii = 4 * Rr + l;
+                const long long off = (long long)ii * ny + c;
+                ga[Tc][Rr] = (wx1[ii] * w1c) * (p1[off] - f1bar);
+                gb[Tc][Rr] = (wx2[ii] * w2c) * (p2[off] - f2bar);
+                m1 = fmax(m1, fabs(ga[Tc][Rr]));
+                m2 = fmax(m2, fabs(gb[Tc][Rr]));
+              });
             });
-            quad_dit<M, -1>(z, l);
-            static_for<M>([&](auto S) { Z[zidx<M>(kb + S, c)] = z[S]; });
+            m1 = fmax(m1, dpp<kQuadXor1>(m1));
+            m2 = fmax(m2, dpp<kQuadXor1>(m2));
+            m1 = fmax(m1, dpp<kQuadXor2>(m1));
+            m2 = fmax(m2, dpp<kQuadXor2>(m2));
+            if (l == 0) {
+              atomicMax(&gmx[0], dbits(m1));
+              atomicMax(&gmx[1], dbits(m2));
+            }
+          }
+          __syncthreads();
+          if (active) {
+            const int l = fresh(lane), quad = fresh(qd);
+            const double g1max = __longlong_as_double((long long)gmx[0]);
+            const double g2max = __longlong_as_double((long long)gmx[1]);
+            const double s1 = (g1max > 0.) ? ldexp(1.0, -ilogb(g1max)) : 1.0;
+            const double s2 = (g2max > 0.) ? ldexp(1.0, -ilogb(g2max)) : 1.0;
+            const int kb = M * brev2(l);
+            static_for<2>([&](auto Tc) {
+              const int c = quad + Tc * (N / 2);
+              cplx<T> z[M];
+              static_for<M>([&](auto Rr) { z[Rr] = {T(ga[Tc][Rr] * s1), T(gb[Tc][Rr] * s2)}; });
+              quad_dit<M, -1>(z, l);
+              static_for<M>([&](auto S) { Z[zidx<M>(kb + S, c)] = z[S]; });
+            });
           }
         }
       }

@@ -23,7 +23,7 @@ PAIRS = [
     (256, 256, 12.0, {"precision": "split_bf16"}),               # row-segment kernel, N = 48
     (180, 190, 11.0, {"thresh": 0.2, "precision": "split_bf16"}),  # relative threshold list
     (130, 140, 4.0, {"skip": 3, "poff": 1, "interp": True}),     # skip lattice and fill
-    (256, 256, 20.0, {"precision": "split_bf16"}),               # dense-DFT kernel, N = 84
+    (256, 256, 20.0, {"precision": "split_bf16"}),               # N = 84: the FP32 kernels
     (220, 200, 9.0, {"thresh": 0.4, "thresh_abs": True, "bias": True}),  # absolute threshold, bias
 ]
 

@@ -18,9 +18,10 @@ pytestmark = pytest.mark.gpu
 RMS_TOL = 1e-4   # px, RMS over all tracked pixels
 MAX_TOL = 5e-3   # px, any pixel
 
-# (n, sigma, options): box sizes 20 and 84 use the dense-DFT kernel; 36..64
-# the row-segment kernel for interior columns and the radix-2 kernel for the
-# clamped edge columns and the bias pass
+# (n, sigma, options): boxes up to 64 use the row-segment kernel for interior
+# columns and a per-pixel kernel (radix-2 for 36..64, dense DFT below) for
+# the clamped edge columns and the bias pass; box 84 runs the FP32 kernels
+# by default (test_bf16_dense_kernel_large_boxes covers the BF16 kernel there)
 CASES = [
     (192, 5.0, {}),                         # N = 20
     (256, 9.0, {}),                         # N = 36
@@ -51,6 +52,25 @@ def test_bf16_matches_fp64(ctx, n, sigma, kw):
     rms, mx = _errs(ref, got)
     assert rms <= RMS_TOL, (rms, mx)
     assert mx <= MAX_TOL, (rms, mx)
+
+
+@pytest.mark.parametrize("n,sigma", [(256, 20.0), (256, 17.0), (192, 5.0)])
+def test_bf16_dense_kernel_large_boxes(monkeypatch, n, sigma):
+    """The per-pixel dense-DFT BF16 kernel (boxes up to 96), which the
+    untuned path rule does not pick above 64, against FP64."""
+    from pyflct_amd import api
+    monkeypatch.setenv("FLCT_HIP_BF16_KERNEL", "pixel")
+    c = api.Context(0)
+    try:
+        f1, f2, _ = synth.cfg4(n=n)
+        ref = c.track(f1, f2, 1.0, 1.0, sigma)
+        got = c.track(f1, f2, 1.0, 1.0, sigma, precision="split_bf16")
+        assert c.last_path() == "bf16-pixel"
+        assert np.array_equal(got[2], ref[2])
+        rms, mx = _errs(ref, got)
+        assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
+    finally:
+        c.close()
 
 
 def test_bf16_refinement_settles_outliers(ctx):
@@ -155,13 +175,18 @@ def test_paths_and_autotune():
         assert c.last_path() == "fp64-fft"
         c.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
         assert c.last_path() == "bf16-segment"
-        c.track(f1, f2, 1.0, 1.0, 20.0, precision="split_bf16")  # box 84
-        assert c.last_path() == "fp64-fft"
+        b84 = c.track(f1, f2, 1.0, 1.0, 20.0, precision="split_bf16")  # box 84
+        assert c.last_path() == "fp32-fft"
+        r84 = c.track(f1, f2, 1.0, 1.0, 20.0)
+        assert np.array_equal(b84[2], r84[2])
+        rms, mx = _errs(r84, b84)
+        assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
         d = [c.malloc(f1.nbytes) for _ in range(2)]
         try:
             c.upload(d[0], f1)
             c.upload(d[1], f2)
-            for form, sigma, allowed in (("split_bf16", 15.0, {"bf16-segment", "bf16-pixel", "fp64-fft"}),
+            for form, sigma, allowed in (("split_bf16", 15.0, {"bf16-segment", "bf16-pixel", "fp32-fft",
+                                                               "fp64-fft"}),
                                          ("fp32", 9.0, {"fp32-fft", "fp64-fft"}),
                                          ("fp64", 12.0, {"fp64-fft"})):
                 best = c.autotune(d[0], d[1], 384, 384, c.make_opts(1.0, 1.0, sigma, precision=form))
