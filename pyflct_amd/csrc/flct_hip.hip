@@ -275,8 +275,8 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
 
 // Untuned choice: the split-BF16 request on boxes above 64 runs the FP32
 // kernels (with their FP64 re-evaluation of unsettled pixels), which are
-// within the split-BF16 error bound and measured 7-16 % faster than the
-// FP64 kernels and about 2.5x faster than the dense-DFT BF16 kernel there
+// within the split-BF16 error bound and measured 20-32 % faster than the
+// FP64 kernels and about 2.7x faster than the dense-DFT BF16 kernel there
 // (2048^2, boxes 68..84).
 static int default_path(int precision, int N, int nx, int ny, int bias, int bf16_kernel) {
   if (precision == FLCT_HIP_FP32) return PATH_FP32;

@@ -93,6 +93,8 @@ struct TrackCfg {
   static constexpr int P = pick();
   static constexpr int kThreads = ((P * kSlotThreads + 63) / 64) * 64;
   static constexpr int kLdsBytes = P * kSlotBytes + (2 * N) * 8;
+  // minimum waves per SIMD asked of the register allocator (1: no bound)
+  static constexpr int kMinWaves = (sizeof(T) == 4 && M >= 18 && M <= 21) ? 2 : 1;
 };
 
 // LDS index of spectrum element (k, c).  Rows of output block p = k / M
@@ -153,7 +155,8 @@ __device__ __forceinline__ cplx<T> cross_power(cplx<T> A, cplx<T> B, T norm) {
 // HIRES (double only): the -h peak step replaces the 3 x 3 fit; a separate
 // instantiation, so the plain kernels keep their register allocation
 template <int M, class T, bool HIRES = false>
-__global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs a) {
+__global__ void __launch_bounds__((TrackCfg<M, T>::kThreads), (TrackCfg<M, T>::kMinWaves))
+    k_track(TrackArgs a) {
   static_assert(!HIRES || sizeof(T) == 8, "the hires step runs in the double kernels");
   using Cfg = TrackCfg<M, T>;
   constexpr int N = 4 * M;
@@ -310,12 +313,13 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads)) k_track(TrackArgs 
           }
         }
       } else {
-        // Boxes 44..68: two passes over the box, the maxima first, then the
+        // Boxes 44..84: two passes over the box, the maxima first, then the
         // scaled values recomputed with the same operations (nothing held
-        // across the barrier: one more wave per SIMD).  Otherwise the values
-        // stay in registers between the passes (no second read of the box;
-        // from 72 up the kernel is at one wave per SIMD either way).
-        if constexpr (M >= 11 && M <= 17) {
+        // across the barrier: two waves per SIMD instead of one; boxes 72..84
+        // also bound the allocator to two waves, spilling a few registers).
+        // Otherwise the values stay in registers between the passes (no
+        // second read of the box).
+        if constexpr (M >= 11 && M <= 21) {
           if (active) {
             const int l = fresh(lane), quad = fresh(qd);
             const double* p1 = a.f1 + (long long)imin * ny + jmin;
