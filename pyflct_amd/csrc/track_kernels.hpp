@@ -93,8 +93,12 @@ struct TrackCfg {
   static constexpr int P = pick();
   static constexpr int kThreads = ((P * kSlotThreads + 63) / 64) * 64;
   static constexpr int kLdsBytes = P * kSlotBytes + (2 * N) * 8;
-  // minimum waves per SIMD asked of the register allocator (1: no bound)
-  static constexpr int kMinWaves = (sizeof(T) == 4 && M >= 18 && M <= 21) ? 2 : 1;
+  // minimum waves per SIMD asked of the register allocator (1: no bound).
+  // Two where the few registers it spills cost less than the second wave
+  // gains (measured at 2048^2): FP32 boxes 72..84, FP64 boxes 40..52.  At
+  // FP64 boxes 56..64 the spills (80..120 registers) made it 1.6x slower.
+  static constexpr int kMinWaves =
+      ((sizeof(T) == 4 && M >= 18 && M <= 21) || (sizeof(T) == 8 && M >= 10 && M <= 13)) ? 2 : 1;
 };
 
 // LDS index of spectrum element (k, c).  Rows of output block p = k / M
