@@ -93,12 +93,16 @@ struct TrackCfg {
   static constexpr int P = pick();
   static constexpr int kThreads = ((P * kSlotThreads + 63) / 64) * 64;
   static constexpr int kLdsBytes = P * kSlotBytes + (2 * N) * 8;
-  // minimum waves per SIMD asked of the register allocator (1: no bound).
-  // Two where the few registers it spills cost less than the second wave
-  // gains (measured at 2048^2): FP32 boxes 72..84, FP64 boxes 40..52.  At
-  // FP64 boxes 56..64 the spills (80..120 registers) made it 1.6x slower.
+  // minimum waves per SIMD asked of the register allocator (1: no bound),
+  // where the registers it spills cost less than the extra wave gains
+  // (measured at 2048^2): three for FP64 box 24, two for FP64 boxes 40..52
+  // and FP32 boxes 72..84.  Elsewhere the bound was slower: FP64 boxes 28
+  // and 32 at three waves (1.5x), FP64 56..64 at two (1.6x), FP32 32..36 at
+  // three.
   static constexpr int kMinWaves =
-      ((sizeof(T) == 4 && M >= 18 && M <= 21) || (sizeof(T) == 8 && M >= 10 && M <= 13)) ? 2 : 1;
+      (sizeof(T) == 8 && M == 6) ? 3
+      : ((sizeof(T) == 4 && M >= 18 && M <= 21) || (sizeof(T) == 8 && M >= 10 && M <= 13)) ? 2
+                                                                                         : 1;
 };
 
 // LDS index of spectrum element (k, c).  Rows of output block p = k / M
