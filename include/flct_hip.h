@@ -161,6 +161,13 @@ const char *flct_hip_last_path(flct_hip_ctx *ctx);
 int flct_hip_autotune(flct_hip_ctx *ctx, const double *d_f1, const double *d_f2, int nx, int ny,
                       const flct_hip_opts *opts);
 
+/* Pins the kernel path of one precision form and box size in the context,
+ * as flct_hip_autotune would: path is one of the names flct_hip_last_path
+ * returns and must meet the form's error contract (FLCT_HIP_E_ARG
+ * otherwise); NULL or "" restores the untuned default.  No counterpart in
+ * the reference. */
+int flct_hip_set_path(flct_hip_ctx *ctx, int precision, int nbox, const char *path);
+
 /*
  * flct_hip_warp replaces warp_frac2d (warp.c:883-1011): out(i,j) is the
  * image's Fourier series evaluated at (i - shiftx, j - shifty), where with

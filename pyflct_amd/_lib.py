@@ -46,6 +46,7 @@ EXPORTS = (
     "flct_hip_last_kernel_ms",
     "flct_hip_last_path",
     "flct_hip_autotune",
+    "flct_hip_set_path",
     "flct_hip_warp",
     "flct_hip_warp_device",
     "flct_hip_set_warp_method",
@@ -130,6 +131,7 @@ def lib():
         L.flct_hip_last_path.argtypes = [vp]
         L.flct_hip_last_path.restype = ctypes.c_char_p
         L.flct_hip_autotune.argtypes = [vp, vp, vp, i, i, ctypes.POINTER(Opts)]
+        L.flct_hip_set_path.argtypes = [vp, i, i, ctypes.c_char_p]
         L.flct_hip_warp.argtypes = [vp, _dp, _dp, _dp, _dp, i, i, i, i]
         L.flct_hip_warp_device.argtypes = [vp, vp, vp, vp, vp, i, i, i]
         L.flct_hip_set_warp_method.argtypes = [vp, i]

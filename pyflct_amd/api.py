@@ -222,6 +222,14 @@ class Context:
                                            ctypes.byref(opts)), "flct_hip_autotune")
         return self.last_path()
 
+    def set_path(self, precision, nbox, path):
+        """Pin the kernel path of a precision form and box size
+        (flct_hip_set_path); ``path=None`` restores the default."""
+        prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        check(_lib.lib().flct_hip_set_path(self._h, prec, int(nbox),
+                                           None if path is None else path.encode()),
+              "flct_hip_set_path")
+
     def last_path(self):
         """Kernel path of the last track call ("fp64-fft", "fp32-fft",
         "bf16-segment", "bf16-pixel")."""

@@ -273,7 +273,9 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
   return {PATH_FP64};
 }
 
-// Untuned choice: the split-BF16 request on boxes above 64 runs the FP32
+// Untuned choice: the split-BF16 request on boxes up to 16 runs the FP32
+// kernels (2048^2, box 16: 9.4 ms against 14.5 ms for the row-segment
+// kernel, whose cost is flat below box 32).  On boxes above 64 it runs the FP32
 // kernels (with their FP64 re-evaluation of unsettled pixels), which are
 // within the split-BF16 error bound and measured 20-32 % faster than the
 // FP64 kernels and about 2.7x faster than the dense-DFT BF16 kernel there
@@ -282,7 +284,7 @@ static int default_path(int precision, int N, int nx, int ny, int bias, int bf16
   if (precision == FLCT_HIP_FP32) return PATH_FP32;
   if (precision == FLCT_HIP_SPLIT_BF16) {
     if (bf16_kernel) return PATH_BF16_PIXEL;
-    if (N > 64) return PATH_FP32;
+    if (N > 64 || N <= 16) return PATH_FP32;
     return seg_supported(N, nx, ny, bias) ? PATH_BF16_SEG : PATH_BF16_PIXEL;
   }
   return PATH_FP64;
@@ -821,6 +823,29 @@ extern "C" int flct_hip_track(flct_hip_ctx* c, const double* f1, const double* f
 
 extern "C" const char* flct_hip_last_path(flct_hip_ctx* c) {
   return c ? path_name(c->last_path) : "none";
+}
+
+extern "C" int flct_hip_set_path(flct_hip_ctx* c, int precision, int nbox, const char* path) {
+  if (!c || nbox < 4 || nbox % 4 != 0 || nbox / 4 > kTrackMaxM) return FLCT_HIP_E_ARG;
+  if (precision < FLCT_HIP_FP64 || precision > FLCT_HIP_SPLIT_BF16) return FLCT_HIP_E_ARG;
+  const std::pair<int, int> key{precision, nbox};
+  if (!path || !*path) {  // back to the untuned default
+    c->tuned.erase(key);
+    ++c->tune_gen;
+    return FLCT_HIP_OK;
+  }
+  int p = -1;
+  for (int q = PATH_FP64; q <= PATH_BF16_PIXEL; ++q)
+    if (std::strcmp(path, path_name(q)) == 0) p = q;
+  if (p < 0) return FLCT_HIP_E_ARG;
+  // the path must meet the form's error contract; the row-segment kernel
+  // also needs a box of 8..64 (calls it cannot serve run the per-pixel one)
+  bool ok = false;
+  for (int q : path_candidates(precision, nbox, 1 << 20, 1 << 20, 0, 0)) ok = ok || (q == p);
+  if (!ok) return FLCT_HIP_E_ARG;
+  c->tuned[key] = p;
+  ++c->tune_gen;
+  return FLCT_HIP_OK;
 }
 
 extern "C" int flct_hip_autotune(flct_hip_ctx* c, const double* f1, const double* f2, int nx,

@@ -129,7 +129,13 @@ def test_bf16_segment_shapes(ctx, nx, ny, sigma, kw):
     from conftest import smooth_pair
     a, b = smooth_pair(rng, nx, ny, shift=(0.7, -1.2), width=2.5)
     ref = ctx.track(a, b, 1.0, 1.0, sigma, **kw)
-    got = ctx.track(a, b, 1.0, 1.0, sigma, precision="split_bf16", **kw)
+    nbox = ctx.box_size(nx, ny, sigma)
+    ctx.set_path("split_bf16", nbox, "bf16-segment")  # also below box 20, where FP32 is the default
+    try:
+        got = ctx.track(a, b, 1.0, 1.0, sigma, precision="split_bf16", **kw)
+        assert ctx.last_path() == "bf16-segment"
+    finally:
+        ctx.set_path("split_bf16", nbox, None)
     assert np.array_equal(got[2], ref[2])
     m = ref[2] == 1.0
     rms, mx = _errs((ref[0][m], ref[1][m]), (got[0][m], got[1][m]))
@@ -162,6 +168,26 @@ def test_bf16_segment_matches_pixel_kernel(monkeypatch):
     assert np.array_equal(g[2], p[2])
     rms, mx = _errs(g, p)
     assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
+
+
+def test_set_path_rules(ctx):
+    from pyflct_amd._lib import FlctHipError
+    for args in (("fp64", 64, "bf16-segment"), ("fp32", 64, "bf16-pixel"), ("split_bf16", 66, "fp64-fft"),
+                 ("split_bf16", 64, "no-such-path"), ("split_bf16", 84, "bf16-segment")):
+        with pytest.raises(FlctHipError):
+            ctx.set_path(*args)
+    f1, f2, _ = synth.cfg4(n=192)
+    ctx.set_path("split_bf16", 64, "fp64-fft")
+    try:
+        got = ctx.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
+        assert ctx.last_path() == "fp64-fft"
+        ref = ctx.track(f1, f2, 1.0, 1.0, 15.0)
+        for x, y in zip(got, ref):
+            assert np.array_equal(x, y)
+    finally:
+        ctx.set_path("split_bf16", 64, None)
+    ctx.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
+    assert ctx.last_path() == "bf16-segment"
 
 
 def test_paths_and_autotune():
