@@ -273,9 +273,9 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
   return {PATH_FP64};
 }
 
-// Untuned choice: the split-BF16 request on boxes up to 16 runs the FP32
-// kernels (2048^2, box 16: 9.4 ms against 14.5 ms for the row-segment
-// kernel, whose cost is flat below box 32).  On boxes above 64 it runs the FP32
+// Untuned choice: the split-BF16 request on boxes up to 20 runs the FP32
+// kernels (2048^2: 7.2 and 12.7 ms at boxes 16 and 20 against 14.5 and
+// 14.6 ms for the row-segment kernel, whose cost is flat below box 32).  On boxes above 64 it runs the FP32
 // kernels (with their FP64 re-evaluation of unsettled pixels), which are
 // within the split-BF16 error bound and measured 20-32 % faster than the
 // FP64 kernels and about 2.7x faster than the dense-DFT BF16 kernel there
@@ -284,7 +284,7 @@ static int default_path(int precision, int N, int nx, int ny, int bias, int bf16
   if (precision == FLCT_HIP_FP32) return PATH_FP32;
   if (precision == FLCT_HIP_SPLIT_BF16) {
     if (bf16_kernel) return PATH_BF16_PIXEL;
-    if (N > 64 || N <= 16) return PATH_FP32;
+    if (N > 64 || N <= 20) return PATH_FP32;
     return seg_supported(N, nx, ny, bias) ? PATH_BF16_SEG : PATH_BF16_PIXEL;
   }
   return PATH_FP64;
