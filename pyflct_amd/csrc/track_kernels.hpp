@@ -134,12 +134,12 @@ struct TrackCfg {
   // minimum waves per SIMD asked of the register allocator (1: no bound),
   // where the registers it spills cost less than the extra wave gains
   // (measured at 2048^2): three for FP64 box 24, two for FP64 boxes 40..52
-  // and FP32 boxes 72..84.  Elsewhere the bound was slower: FP64 boxes 28
+  // and FP32 boxes 72..96.  Elsewhere the bound was slower: FP64 boxes 28
   // and 32 at three waves (1.5x), FP64 56..64 at two (1.6x), FP32 32..36 at
   // three.
   static constexpr int kMinWaves =
       (sizeof(T) == 8 && M == 6) ? 3
-      : ((sizeof(T) == 4 && M >= 18 && M <= 21) || (sizeof(T) == 8 && M >= 10 && M <= 13)) ? 2
+      : ((sizeof(T) == 4 && M >= 18 && M <= 24) || (sizeof(T) == 8 && M >= 10 && M <= 13)) ? 2
                                                                                          : 1;
 };
 
@@ -359,13 +359,13 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads), (TrackCfg<M, T>::k
           }
         }
       } else {
-        // Boxes 44..84: two passes over the box, the maxima first, then the
+        // Boxes 44..96: two passes over the box, the maxima first, then the
         // scaled values recomputed with the same operations (nothing held
-        // across the barrier: two waves per SIMD instead of one; boxes 72..84
+        // across the barrier: two waves per SIMD instead of one; boxes 72..96
         // also bound the allocator to two waves, spilling a few registers).
         // Otherwise the values stay in registers between the passes (no
         // second read of the box).
-        if constexpr (M >= 11 && M <= 21) {
+        if constexpr (M >= 11 && M <= 24) {
           if (active) {
             const int l = fresh(lane), quad = fresh(qd);
             const double* p1 = a.f1 + (long long)imin * ny + jmin;
