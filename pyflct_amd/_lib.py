@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libflct_hip.so")
+# FLCT_HIP_LIB selects another build of the library (tuning variants)
+LIB_PATH = os.environ.get("FLCT_HIP_LIB") or os.path.join(HERE, "libflct_hip.so")
 
 FLCT_HIP_OK = 0
 FLCT_HIP_E_ARG = 1

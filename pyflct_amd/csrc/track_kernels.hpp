@@ -66,8 +66,9 @@ constexpr double kRelErr32 = 1e-6;
 constexpr double kFitTol32 = 1e-2;
 
 // Slots (pixels tracked together) per block, from a sweep of 1, 2, 3, 4, 6
-// and 8 slots over every box size at 2048^2 (every pixel tracked), where the
-// best count beat the default rule below by 5 % or more; 0: default rule.
+// and 8 slots over every box size at 2048^2 (every pixel tracked,
+// pyflct_amd/tune/sweep_slots.py), where the best count beat the default
+// rule below by 4 % or more; 0: default rule.
 constexpr int tuned_slots(int M, bool fp32) {
   if (fp32) {
     switch (M) {
@@ -127,8 +128,12 @@ struct TrackCfg {
     while (p > 1 && (p * kSlotBytes + 2 * N * 8 > 160 * 1024 || p * kSlotThreads > 1024)) --p;
     return p;
   }
+#ifdef FLCT_TRACK_SLOTS
+  static constexpr int P = fit(FLCT_TRACK_SLOTS);  // tuning builds: one count for every size
+#else
   static constexpr int P = tuned_slots(M, sizeof(T) == 4) > 0 ? fit(tuned_slots(M, sizeof(T) == 4))
                                                              : pick();
+#endif
   static constexpr int kThreads = ((P * kSlotThreads + 63) / 64) * 64;
   static constexpr int kLdsBytes = P * kSlotBytes + (2 * N) * 8;
   // minimum waves per SIMD asked of the register allocator (1: no bound),

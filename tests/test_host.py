@@ -96,3 +96,15 @@ def test_vcimage_round_trip_and_header(tmp_path, golden):
     bad.write_bytes(b"\0" * 16)
     with pytest.raises(vcimage.VcimageError):
         vcimage.read(bad)
+
+
+def test_slot_sweep_table(tmp_path, capsys):
+    """The slot sweep's table step keeps a count only where it beats the
+    default build by the margin."""
+    from pyflct_amd.tune import sweep_slots
+    f = tmp_path / "sweep.txt"
+    f.write_text("default fp32 28 35.5\nlibflct_hip_p1.so fp32 28 24.9\nlibflct_hip_p2.so fp32 28 26.9\n"
+                 "default fp64 64 212.1\nlibflct_hip_p4.so fp64 64 211.2\n# comment\n")
+    sweep_slots.table(str(f), 0.04)
+    out = capsys.readouterr().out.strip().splitlines()
+    assert out == ["fp32 box 28 (M = 7): 1 slots, 35.5 -> 24.9 ms"]
