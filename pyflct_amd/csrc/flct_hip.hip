@@ -277,8 +277,8 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
 // kernels (2048^2: 7.2 and 12.7 ms at boxes 16 and 20 against 14.5 and
 // 14.6 ms for the row-segment kernel, whose cost is flat below box 32).  On boxes above 64 it runs the FP32
 // kernels (with their FP64 re-evaluation of unsettled pixels), which are
-// within the split-BF16 error bound and measured 20-32 % faster than the
-// FP64 kernels and about 2.7x faster than the dense-DFT BF16 kernel there
+// within the split-BF16 error bound and measured 29-41 % faster than the
+// FP64 kernels and 2.9-3.7x faster than the dense-DFT BF16 kernel there
 // (2048^2, boxes 68..84).
 static int default_path(int precision, int N, int nx, int ny, int bias, int bf16_kernel) {
   if (precision == FLCT_HIP_FP32) return PATH_FP32;
