@@ -22,6 +22,7 @@ METRIC = "tracked px/s at 4096^2 sigma=15: split-BF16 MFMA vs FP64 plain; RMS ve
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
+PEAK_LDS_TBS = 128 * 256 * 2.4e9 / 1e12  # 128 B per clock per CU, 256 CUs, 2.4 GHz
 PMC_FILE = os.path.join(ROOT, "profiles", "track_4096_sigma15_traffic.json")
 
 
@@ -38,6 +39,19 @@ def w_gemm(n):
     inverse 8 N^2 (N/2+1) + 4 N^3, times 3 for hi*hi + hi*lo + lo*hi."""
     h = n // 2 + 1
     return 3.0 * (2 * (4 * n ** 3 + 8 * n * n * h) + (8 * n * n * h + 4 * n ** 3))
+
+
+def lds_bytes_fft(n, elem=16, window_reads=1):
+    """LDS bytes per tracked pixel of the FFT tracker k_track (complex
+    element of elem bytes, window values double), counted from its access
+    pattern: phase 1 stores the N x N packed column spectra and reads the two
+    row-window values of every box element; phase 2 reads rows k and N-k,
+    parks and rereads A's spectrum and stores the product's inverse
+    (5N elements per row pair, 10N for rows 0 and N/2); phase 3 reads the
+    N/2+1 stored rows as N columns.  4.5 N^2 + 5N elements plus 2 N^2
+    doubles per window pass (the FP32 kernels for boxes 44..96 make two:
+    the box maxima, then the scaled values)."""
+    return (4.5 * n * n + 5 * n) * elem + window_reads * 2 * n * n * 8
 
 
 MFMA_FLOPS = 2 * 16 * 16 * 32  # one v_mfma_f32_16x16x32_bf16
@@ -155,6 +169,33 @@ def main():
     if "fp64" not in results:
         raise SystemExit("bench: the FP64 plain form failed")
     nbox = ctx.box_size(nx, ny, args.sigma)
+    # LDS bandwidth of each form's tracking step: bytes counted by rocprofv3
+    # (SQ_INSTS_LDS_LOAD_BANDWIDTH + SQ_INSTS_LDS_STORE_BANDWIDTH, 64 B units)
+    # on this workload when the counter file holds them, beside the access
+    # pattern model of the FFT kernels.  The counter file describes the
+    # 4096^2, sigma = 15 pair only.
+    pmc = {}
+    if os.path.exists(PMC_FILE) and nbox == 64 and (nx, ny) == (4096, 4096):
+        try:
+            pmc = json.load(open(PMC_FILE))
+        except (OSError, ValueError):
+            pmc = {}
+    lds = {}
+    for form, r in results.items():
+        counted = pmc.get(form, {}).get("lds_bytes_per_launch")
+        model = None
+        if form in ("fp64", "fp32"):
+            model = lds_bytes_fft(nbox, 16 if form == "fp64" else 8,
+                                  2 if (form == "fp32" and 44 <= nbox <= 96) else 1) * r["tracked"]
+        b = counted if counted else model
+        if not b:
+            continue
+        tbs = b / (r["main_ms"] * 1e-3) / 1e12
+        lds[form] = {"bytes_per_px": round(b / r["tracked"]), "source": "rocprofv3 counters" if counted
+                     else "access-pattern model", "achieved_TBps": round(tbs, 2),
+                     "peak_TBps": round(PEAK_LDS_TBS, 1), "frac": round(tbs / PEAK_LDS_TBS, 4)}
+        if model and counted:
+            lds[form]["model_bytes_per_px"] = round(model / r["tracked"])
     for form, r in results.items():
         r["rms_vel_err_vs_fp64"] = float(np.sqrt(np.mean((r["vx"] - ref[0]) ** 2 +
                                                          (r["vy"] - ref[1]) ** 2)))
@@ -175,6 +216,18 @@ def main():
     if "split_bf16" in results and results["split_bf16"]["rms_vel_err_vs_fp64"] <= 1e-4:
         head = "split_bf16"
     hr = results[head]
+
+    # end to end through the host-buffer entry point: H2D of both images
+    # (pageable float64), the tracking step, D2H of vx, vy, vm
+    e2e = []
+    for _ in range(2):
+        t = time.perf_counter()
+        ctx.track(f1, f2, 1.0, 1.0, args.sigma, precision=head)
+        e2e.append((time.perf_counter() - t) * 1e3)
+    end_to_end = {"form": head, "ms": round(e2e[-1], 3),
+                  "px_per_s": round(hr["tracked"] / (e2e[-1] * 1e-3), 1),
+                  "buffers": "host float64 (pageable): 2 images in, vx, vy, vm out"}
+    log(f"bench: end to end {e2e[-1]:.1f} ms")
     dtype = {"fp64": "f64", "fp32": "f32", "split_bf16": "bf16x3+f32acc"}[head]
 
     if head == "fp64":
@@ -187,12 +240,7 @@ def main():
                  if 36 <= nbox <= 64 else "k_track_mfma")
     flops = per_px * hr["tracked"]
     achieved = flops / (hr["main_ms"] * 1e-3) / 1e12
-    traffic = None
-    if os.path.exists(PMC_FILE):
-        try:
-            traffic = json.load(open(PMC_FILE)).get(head, {}).get("hbm_bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
+    traffic = pmc.get(head, {}).get("hbm_bytes_per_launch")
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": kname,
             "flops_per_px": per_px, "kernel_ms": round(hr["main_ms"], 3)}
@@ -244,6 +292,8 @@ def main():
                       "max_vel_err_vs_fp64": r["max_vel_err_vs_fp64"]}
                   for f, r in results.items()},
         "fp64_vs_oracle_max_err_px": parity,
+        "end_to_end": end_to_end,
+        "lds": lds,
         "roofline": roof,
         "cpu_baseline": cpu,
         "warp": warp,
