@@ -42,7 +42,11 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 template <int PIX>
 struct R2Cfg {
-  static constexpr int RS = 40;              // bf16 row stride of 32-wide rows (80 B)
+  // bf16 row stride of 32-wide rows: 40 (80 B).  48 makes the fragment loads
+  // free of bank conflicts (as in the row-segment kernel) but measured 4 %
+  // slower here: 4.81 against 4.62 ms over the edge columns of a 4096^2
+  // pair with 64 x 64 boxes
+  static constexpr int RS = 40;
   static constexpr int kTabPlane = 32 * RS;  // one plane of a 32-row table
   static constexpr int kTabEl = 4 * kTabPlane;
   static constexpr int kPixPlane = 64 * RS;  // one plane of G or U
@@ -53,6 +57,7 @@ struct R2Cfg {
   static constexpr int kPixStride = (kPixBytes + 15) / 16 * 16;
   static constexpr int kThreads = PIX * 128;
   static constexpr int kLdsBytes = 2 * kTabEl * 2 + 32 * 8 + PIX * kPixStride;
+  static_assert(kLdsBytes <= 80 * 1024, "two blocks of k_track_r2 per CU");
 };
 
 // row permutation of the stage-2 / inverse-1 outputs: r = 16a + 4g + b

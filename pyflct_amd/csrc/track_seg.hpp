@@ -72,7 +72,12 @@ struct SegCfg {
   static constexpr int kWaves = 8;
 #endif
   static constexpr int kThreads = 64 * kWaves;
-  static constexpr int RS = 40;           // table row stride (bf16), 32 + 8
+  // table row stride (bf16): 48 = 24 dwords.  A fragment load is a
+  // ds_read_b128 of row n = lane & 15 at dword 4 (lane >> 4); measured on
+  // gfx950 (SQ_LDS_BANK_CONFLICT), row strides of 24 or 40 dwords serve it
+  // without bank conflicts, 20 dwords (the padded minimum) costs 4 conflict
+  // cycles per load
+  static constexpr int RS = 48;
   static constexpr int kPlane = 32 * RS;
   static constexpr int RS1 = 72;          // stage-1 table row stride, 64 + 8
   static constexpr int kD1Plane = 32 * RS1;
@@ -94,6 +99,7 @@ struct SegCfg {
   static constexpr int kTEl = 4 * kTPlane;
   static constexpr int kScratch = 128 * 8 + 64 * 8 + 9 * 8 + 8;  // zx, zp, nb, pad
   static constexpr int kLdsBytes = kTabEl * 2 + 256 * 4 + kTEl * 2 + 68 * 4 + kWaves * kScratch;
+  static_assert(kLdsBytes <= 160 * 1024, "k_track_seg needs more LDS than a CU has");
 };
 
 __device__ __forceinline__ bf16x8 u4_to_bf8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
