@@ -133,16 +133,17 @@ int flct_hip_track_batch(flct_hip_ctx *ctx, flct_hip_pair *pairs, int npairs);
 int flct_hip_last_stats(flct_hip_ctx *ctx, long long *tracked, double *kernel_ms,
                         double *total_ms);
 
-/* Number of pixels the last FLCT_HIP_SPLIT_BF16 call re-evaluated in FP64
- * because the BF16 correlation surface could not settle them (a second cell
- * within 1e-4 of the maximum, or an ill-conditioned peak fit); 0 for the
- * other forms.  No counterpart in the reference, which has one precision. */
+/* Number of pixels the last FLCT_HIP_SPLIT_BF16 or FLCT_HIP_FP32 call
+ * re-evaluated in FP64 because its correlation surface could not settle them
+ * (a second cell within 1e-4 (split-BF16) or 1e-5 (FP32) of the maximum, or
+ * an ill-conditioned peak fit); 0 for FLCT_HIP_FP64.  No counterpart in the
+ * reference, which has one precision. */
 int flct_hip_last_refined(flct_hip_ctx *ctx, long long *refined);
 
 /* Device time in milliseconds of the last track call's tracking kernel alone
  * (main_ms) and of the FP64 re-evaluation of flagged pixels (refine_ms, 0
- * except for FLCT_HIP_SPLIT_BF16); HIP events on the context's stream.  No
- * counterpart in the reference. */
+ * for FLCT_HIP_FP64); HIP events on the context's stream.  No counterpart in
+ * the reference. */
 int flct_hip_last_kernel_ms(flct_hip_ctx *ctx, double *main_ms, double *refine_ms);
 
 /* Kernel path of the last track call: "fp64-fft", "fp32-fft",
