@@ -106,9 +106,10 @@ def main():
     ap.add_argument("--forms", default="fp64,fp32,split_bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--workload", default="full-disk", choices=["full-disk", "uneven-batch"],
+    ap.add_argument("--workload", default="full-disk", choices=["full-disk", "uneven-batch", "configs"],
                     help="full-disk: the 4096^2 sigma=15 pair (default); uneven-batch: 64 pairs "
-                         "of sizes 256..2048 and sigma 4..20 in one graph launch")
+                         "of sizes 256..2048 and sigma 4..20 in one graph launch; configs: the "
+                         "128^2, 512^2 and 1024^2 cases with their rates and errors")
     ap.add_argument("--pairs", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -121,6 +122,8 @@ def main():
 
     if args.workload == "uneven-batch":
         return uneven_batch(args)
+    if args.workload == "configs":
+        return configs_bench(args)
 
     log(f"bench: generating the {args.size}^2 pair")
     f1, f2, vtrue = synth.cfg4(n=args.size)
@@ -411,6 +414,114 @@ def uneven_batch(args):
         "tracking_plus_warp_ms": round(hr["ms"] + wms, 3),
     }
     print(json.dumps(out))
+
+
+def configs_bench(args):
+    """The three smaller cases, each timed on resident buffers with its error
+    stated: 128^2 blobs shifted by (0.3, -0.2), "1 1 8" (FP64); 512^2
+    magnetogram under a 0.01 degree rotation, "1 1 6 -t 0.1 -k 0.5 -b"
+    (FP64); 1024^2 float32 magnetogram under a smooth random field,
+    "1 1 10 -s 4i", in FP64 and in the FP32 form through one graph launch
+    (flct_hip_track_batch).  FP64 is compared with the CPU oracle (every
+    pixel at 128^2, 2000 sampled tracked pixels otherwise), FP32 with FP64."""
+    from pyflct_amd import api, synth
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    ctx = api.Context(0)
+    rng = np.random.default_rng(5)
+    steps = max(1, args.steps)
+
+    def resident(f1, f2):
+        d = {k: ctx.malloc(f1.nbytes) for k in ("f1", "f2", "vx", "vy", "vm")}
+        ctx.upload(d["f1"], f1)
+        ctx.upload(d["f2"], f2)
+        return d
+
+    def timed(d, nx, ny, opts):
+        ms = []
+        for _ in range(1 + steps):
+            ctx.track_device(d["f1"], d["f2"], nx, ny, d["vx"], d["vy"], d["vm"], opts)
+            ms.append(ctx.last_stats()["kernel_ms"])
+        tracked = ctx.last_stats()["tracked"]
+        out = [ctx.download(np.empty((nx, ny)), d[k]) for k in ("vx", "vy", "vm")]
+        return float(np.mean(ms[1:])), tracked, out
+
+    def vs_oracle(f1, f2, sigma, v, pix, **kw):
+        ox, oy, om = oracle_lib.flct_pixels(f1, f2, 1.0, 1.0, sigma, pix, **kw)
+        return float(max(np.max(np.abs(v[0].ravel()[pix] - ox)), np.max(np.abs(v[1].ravel()[pix] - oy))))
+
+    res = {}
+    # 128^2, "1 1 8"
+    f1, f2 = synth.cfg1()
+    d = resident(f1, f2)
+    ms, tr, v = timed(d, 128, 128, ctx.make_opts(1.0, 1.0, 8.0))
+    o = oracle_lib.flct(f1, f2, 1.0, 1.0, 8.0)
+    res["blobs_128_sigma8"] = {"form": "fp64", "kernel_ms": round(ms, 4), "tracked_px": tr,
+                               "tracked_px_per_s": round(tr / (ms * 1e-3), 1),
+                               "max_err_vs_oracle_px": float(max(np.max(np.abs(v[0] - o[0])),
+                                                                 np.max(np.abs(v[1] - o[1])))),
+                               "vm_equal": bool(np.array_equal(v[2], o[2])),
+                               "mean_v_interior": [float(np.mean(v[0][32:96, 32:96])),
+                                                   float(np.mean(v[1][32:96, 32:96]))]}
+    for k in d.values():
+        ctx.free(k)
+    log(f"bench: 128^2 sigma 8: {ms:.3f} ms")
+    # 512^2, "1 1 6 -t 0.1 -k 0.5 -b"
+    f1, dx, dy = synth.cfg2_inputs()
+    f2 = ctx.warp(f1, dx, dy)
+    d = resident(f1, f2)
+    kw = dict(thresh=0.1, kr=0.5, bias=True)
+    ms, tr, v = timed(d, 512, 512, ctx.make_opts(1.0, 1.0, 6.0, **kw))
+    tracked_idx = np.flatnonzero(v[2].ravel() == 1.0)
+    pix = rng.choice(tracked_idx, min(2000, tracked_idx.size), replace=False)
+    res["magnetogram_512_sigma6_t0.1_k0.5_b"] = {
+        "form": "fp64", "kernel_ms": round(ms, 4), "tracked_px": tr,
+        "tracked_px_per_s": round(tr / (ms * 1e-3), 1),
+        "max_err_vs_oracle_px": vs_oracle(f1, f2, 6.0, v, pix, **kw), "oracle_sample_px": int(pix.size)}
+    for k in d.values():
+        ctx.free(k)
+    log(f"bench: 512^2 sigma 6 -t 0.1 -k 0.5 -b: {ms:.3f} ms, {tr} tracked")
+    # 1024^2 float32, "1 1 10 -s 4i": FP64, then FP32 through one graph launch
+    f1, dx, dy = synth.cfg3_inputs()
+    f1 = f1.astype(np.float32).astype(np.float64)
+    f2 = ctx.warp(f1, dx, dy).astype(np.float32).astype(np.float64)
+    d = resident(f1, f2)
+    kw = dict(skip=4, interp=True)
+    ms64, tr, v64 = timed(d, 1024, 1024, ctx.make_opts(1.0, 1.0, 10.0, **kw))
+    lat = np.flatnonzero(v64[2].ravel() == 1.0)
+    pix = rng.choice(lat, min(2000, lat.size), replace=False)
+    err64 = vs_oracle(f1, f2, 10.0, v64, pix, skip=4)
+    o32 = {k: ctx.malloc(f1.nbytes) for k in ("vx", "vy", "vm")}
+    desc = [(d["f1"], d["f2"], o32["vx"], o32["vy"], o32["vm"], 1024, 1024,
+             ctx.make_opts(1.0, 1.0, 10.0, precision="fp32", **kw))]
+    ms32 = []
+    for _ in range(1 + steps):
+        st = ctx.track_batch_device(desc)
+        ms32.append(ctx.last_stats()["kernel_ms"])
+    v32 = [ctx.download(np.empty((1024, 1024)), o32[k]) for k in ("vx", "vy", "vm")]
+    ms32 = float(np.mean(ms32[1:]))
+    e = np.hypot(v32[0] - v64[0], v32[1] - v64[1])
+    m1, mh = v64[2] == 1.0, v64[2] == 0.5
+    res["magnetogram_1024_f32_sigma10_s4i"] = {
+        "fp64": {"kernel_ms": round(ms64, 4), "tracked_px": tr,
+                 "tracked_px_per_s": round(tr / (ms64 * 1e-3), 1),
+                 "output_px_per_s": round(1024 * 1024 / (ms64 * 1e-3), 1),
+                 "max_err_vs_oracle_px": err64, "oracle_sample_px": int(pix.size)},
+        "fp32_graph": {"kernel_ms": round(ms32, 4), "tracked_px": st[0]["tracked"],
+                       "refined_px": st[0]["refined"],
+                       "tracked_px_per_s": round(st[0]["tracked"] / (ms32 * 1e-3), 1),
+                       "output_px_per_s": round(1024 * 1024 / (ms32 * 1e-3), 1),
+                       "vm_equal_fp64": bool(np.array_equal(v32[2], v64[2])),
+                       "rms_err_vs_fp64_tracked": float(np.sqrt(np.mean(e[m1] ** 2))),
+                       "max_err_vs_fp64_tracked": float(e[m1].max()),
+                       "rms_err_vs_fp64_filled": float(np.sqrt(np.mean(e[mh] ** 2))),
+                       "max_err_vs_fp64_filled": float(e[mh].max())}}
+    for k in list(d.values()) + list(o32.values()):
+        ctx.free(k)
+    log(f"bench: 1024^2 sigma 10 -s 4i: FP64 {ms64:.3f} ms, FP32 graph {ms32:.3f} ms")
+    print(json.dumps({"metric": "tracked px/s and errors of the 128^2, 512^2 and 1024^2 cases",
+                      "n_gpus": 1, "steps": steps, "data": "synthetic (PCG64, see pyflct_amd/synth.py)",
+                      "configs": res}))
 
 
 if __name__ == "__main__":

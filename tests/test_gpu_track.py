@@ -169,6 +169,29 @@ def test_cfg3_skip_fill(ctx):
     assert np.sum(g[2] == 0.5) == 1024 * 1024 - 256 * 256 - np.sum(g[2] == 0)
 
 
+def test_cfg3_fp32_form_in_one_graph_launch(ctx):
+    """The same case with float32 images in the FP32 form, tracked through
+    flct_hip_track_batch (one graph launch), against the FP64 form: the vm
+    mask, the lattice and the filled set are bit-exact; velocities of tracked
+    and filled pixels within RMS 1e-4 px and max 5e-3 px (observed ~1e-6 and
+    ~3e-4 at 4096^2)."""
+    from pyflct_amd import synth
+    f1, dx, dy = synth.cfg3_inputs()
+    f1 = f1.astype(np.float32).astype(np.float64)
+    f2 = ctx.warp(f1, dx, dy).astype(np.float32).astype(np.float64)
+    ref = ctx.track(f1, f2, 1.0, 1.0, 10.0, skip=4, interp=True)
+    (got,) = ctx.track_batch([(f1, f2, 1.0, 1.0, 10.0)], skip=4, interp=True, precision="fp32")
+    assert np.array_equal(got[2], ref[2])
+    e = np.hypot(got[0] - ref[0], got[1] - ref[1])
+    for sel in (ref[2] == 1.0, ref[2] == 0.5):
+        assert sel.any()
+        rms, mx = float(np.sqrt(np.mean(e[sel] ** 2))), float(e[sel].max())
+        print(f"1024^2 f32 -s 4i, FP32 graph vs FP64: rms {rms:.2e} max {mx:.2e} px over {sel.sum()} px")
+        assert rms <= 1e-4 and mx <= 5e-3, (rms, mx)
+    z = ref[2] == 0.0
+    assert np.array_equal(got[0][z], ref[0][z]) and np.array_equal(got[1][z], ref[1][z])
+
+
 def test_cfg4_full_disk_sampled(ctx):
     """4096^2 full-disk pair, "1 1 15": sampled parity (interior, limb with
     the max == 0 branch, frame edges) and recovery of the rotation law."""
