@@ -30,27 +30,62 @@ __global__ void k_absmax(const double* __restrict__ f, long long n,
 // Pass 1: hs[r][s] = sum_{c<N} f[r][s+c]; pass 2: mean[imin][jmin] =
 // (sum_{r<N} hs[imin+r][jmin]) / (N*N), rows summed column-first like the
 // reference's row-major loop.
-__global__ void k_rowsum(const double* __restrict__ f, int nx, int ny, int N,
-                         double* __restrict__ hs) {
+// Each output is summed in the same order as a plain loop over c (pass 1)
+// and r (pass 2), so the means do not depend on the launch shape.
+// Pass 1 stages kMeanTile + N - 1 values of one image row in LDS and each
+// thread sums its N from there; pass 2 gives each thread kMeanRows
+// consecutive box origins of one column and adds every loaded row sum to the
+// origins whose box holds it, so each value is loaded once per thread
+// instead of once per output.
+constexpr int kMeanTile = 256;     // pass-1 outputs per block (= threads)
+constexpr int kMeanRows = 8;       // pass-2 origins per thread
+constexpr int kMeanMaxN = 96;      // largest box
+
+__global__ void __launch_bounds__(kMeanTile) k_rowsum(const double* __restrict__ f, int nx, int ny, int N,
+                                                      double* __restrict__ hs) {
+  __shared__ double tile[kMeanTile + kMeanMaxN - 1];
   const int ns = ny - N + 1;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)nx * ns) return;
-  const int r = (int)(t / ns), sidx = (int)(t - (long long)r * ns);
-  const double* p = f + (long long)r * ny + sidx;
+  const int nbx = (ns + kMeanTile - 1) / kMeanTile;
+  const int r = (int)(blockIdx.x / nbx);
+  const int s0 = (int)(blockIdx.x - (long long)r * nbx) * kMeanTile;
+  const double* p = f + (long long)r * ny;
+  for (int i = threadIdx.x; i < kMeanTile + N - 1; i += kMeanTile) {
+    const int c = s0 + i;
+    tile[i] = (c < ny) ? p[c] : 0.;
+  }
+  __syncthreads();
+  const int sidx = s0 + (int)threadIdx.x;
+  if (sidx >= ns) return;
   double acc = 0.;
-  for (int c = 0; c < N; ++c) acc = acc + p[c];
-  hs[t] = acc;
+  for (int c = 0; c < N; ++c) acc = acc + tile[threadIdx.x + c];
+  hs[(long long)r * ns + sidx] = acc;
 }
 
-__global__ void k_boxmean(const double* __restrict__ hs, int nx, int ny, int N,
-                          double* __restrict__ mean) {
+__global__ void __launch_bounds__(256) k_boxmean(const double* __restrict__ hs, int nx, int ny, int N,
+                                                 double* __restrict__ mean) {
   const int ns = ny - N + 1, nr = nx - N + 1;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)nr * ns) return;
-  const int r = (int)(t / ns), sidx = (int)(t - (long long)r * ns);
-  double acc = 0.;
-  for (int q = 0; q < N; ++q) acc = acc + hs[(long long)(r + q) * ns + sidx];
-  mean[t] = acc / ((double)N * N);
+  const int nbx = (ns + 255) / 256;
+  const int rb = (int)(blockIdx.x / nbx);
+  const int sidx = (int)(blockIdx.x - (long long)rb * nbx) * 256 + (int)threadIdx.x;
+  if (sidx >= ns) return;
+  const int r0 = rb * kMeanRows;
+  double acc[kMeanRows];
+#pragma unroll
+  for (int k = 0; k < kMeanRows; ++k) acc[k] = 0.;
+  const int last = ((r0 + kMeanRows < nr) ? r0 + kMeanRows : nr) - 1 + N - 1;  // last row read
+  // unrolled so that several row loads are in flight per thread
+#pragma unroll 8
+  for (int q = 0; r0 + q <= last; ++q) {
+    const double v = hs[(long long)(r0 + q) * ns + sidx];
+#pragma unroll
+    for (int k = 0; k < kMeanRows; ++k) {
+      // origin r0 + k sums rows r0 + k .. r0 + k + N - 1 in increasing order
+      if (q >= k && q - k < N) acc[k] = acc[k] + v;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kMeanRows; ++k)
+    if (r0 + k < nr) mean[(long long)(r0 + k) * ns + sidx] = acc[k] / ((double)N * N);
 }
 
 struct SelectArgs {

@@ -22,6 +22,8 @@
 
 using namespace flct;
 
+static_assert(4 * kTrackMaxM <= kMeanMaxN, "the box-mean row tile must hold the largest box");
+
 namespace {
 
 
@@ -448,12 +450,13 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
     hipLaunchKernelGGL(k_select, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, sa);
   }
   // box means by box origin (flct.c:815-827)
-  const long long nh = (long long)nx * (ny - N + 1), nm = (long long)(nx - N + 1) * (ny - N + 1);
-  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, f1, nx, ny, N, sc.hs1);
-  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, f2, nx, ny, N, sc.hs2);
-  hipLaunchKernelGGL(k_boxmean, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s,
+  const long long nbx = (ny - N + 1 + kMeanTile - 1) / kMeanTile;
+  const long long nrb = (nx - N + 1 + kMeanRows - 1) / kMeanRows, nby = (ny - N + 1 + 255) / 256;
+  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)(nx * nbx)), dim3(kMeanTile), 0, s, f1, nx, ny, N, sc.hs1);
+  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)(nx * nbx)), dim3(kMeanTile), 0, s, f2, nx, ny, N, sc.hs2);
+  hipLaunchKernelGGL(k_boxmean, dim3((unsigned)(nrb * nby)), dim3(256), 0, s,
                      (const double*)sc.hs1, nx, ny, N, sc.mean1);
-  hipLaunchKernelGGL(k_boxmean, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_boxmean, dim3((unsigned)(nrb * nby)), dim3(256), 0, s,
                      (const double*)sc.hs2, nx, ny, N, sc.mean2);
   HIPCHK(hipGetLastError());
 
