@@ -1296,7 +1296,8 @@ static int warp_nufft(flct_hip_ctx* c, const double* delx, const double* dely, d
                             (int)ldsc);
   hipLaunchKernelGGL(k_nufft_rows, dim3(nx), dim3(kNuThreads), ldsr, s, q);
   hipLaunchKernelGGL(k_nufft_cols, dim3(My), dim3(kNuThreads), ldsc, s, q);
-  hipLaunchKernelGGL(k_nufft_gather, dim3(nblk((long long)nx * ny, 256)), dim3(256), 0, s, q);
+  const long long tiles = (long long)((nx + kNuTile - 1) / kNuTile) * ((ny + kNuTile - 1) / kNuTile);
+  hipLaunchKernelGGL(k_nufft_gather, dim3((unsigned)tiles), dim3(kNuTile * kNuTile), 0, s, q);
   HIPCHK(hipGetLastError());
   return FLCT_HIP_OK;
 }

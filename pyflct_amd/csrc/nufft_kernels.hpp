@@ -131,12 +131,20 @@ __device__ __forceinline__ double es_phi(double z, double beta) {
   return (r > 0.) ? exp(beta * (sqrt(r) - 1.)) : 0.;
 }
 
-// one thread per output pixel
-__global__ void __launch_bounds__(256) k_nufft_gather(NufftArgs q) {
-  const long long npix = (long long)q.nx * q.ny;
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npix) return;
-  const int i = (int)(p / q.ny), j = (int)(p - (long long)i * q.ny);
+// One thread per output pixel; a block takes a tile of kNuTile x kNuTile
+// pixels, i fastest within a wave.  Grid coordinates follow i and j (about
+// 2 cells per pixel), so for one tap the lanes of a wave read a few grid rows
+// at every other column, and a block's taps stay inside a small window of
+// the grid.
+constexpr int kNuTile = 16;
+
+__global__ void __launch_bounds__(kNuTile * kNuTile) k_nufft_gather(NufftArgs q) {
+  const int tiles_i = (q.nx + kNuTile - 1) / kNuTile;
+  const int ti = (int)(blockIdx.x % tiles_i), tj = (int)(blockIdx.x / tiles_i);
+  const int i = ti * kNuTile + (int)(threadIdx.x % kNuTile);
+  const int j = tj * kNuTile + (int)(threadIdx.x / kNuTile);
+  if (i >= q.nx || j >= q.ny) return;
+  const long long p = (long long)i * q.ny + j;
   double shiftx, shifty;
   if (q.transp) {
     shiftx = q.dely[p];
