@@ -228,3 +228,38 @@ def test_paths_and_autotune():
                 c.free(p_)
     finally:
         c.close()
+
+
+# every box size 4..96 in the FP32 and split-BF16 forms (their default kernel
+# paths: FP32 kernels for every box in FP32; in split-BF16 the FP32 kernels up
+# to box 20 and above 64, the row-segment kernel from 24 to 64)
+BOXES = list(range(4, 100, 4))
+
+
+def _box_case(nbox):
+    """(sigma, nx, ny) giving box nbox: an integer sigma whose nt (flct.c:567-580,
+    sigma truncated to an integer) gives it, else sigma 14 with the image's
+    size clamping the box (only box 56 has no integer sigma)."""
+    import math
+    for s in range(1, 24):
+        nt = int(s * math.sqrt(math.log(100.0)))
+        if 2 * ((nt // 2) * 2) == nbox:
+            n = max(64, 3 * nbox + 6)
+            return float(s), n, n + 5
+    return 14.0, 3 * (nbox // 2) + 1, 3 * (nbox // 2) + 6
+
+
+@pytest.mark.parametrize("form", ["fp32", "split_bf16"])
+@pytest.mark.parametrize("nbox", BOXES, ids=[f"N{b}" for b in BOXES])
+def test_forms_every_box_size(ctx, form, nbox):
+    sigma, nx, ny = _box_case(nbox)
+    assert ctx.box_size(nx, ny, sigma) == nbox
+    rng = np.random.default_rng(nbox)
+    from conftest import smooth_pair
+    a, b = smooth_pair(rng, nx, ny, shift=(0.6, -0.9), width=max(1.5, nbox / 16))
+    ref = ctx.track(a, b, 1.0, 1.0, sigma)
+    got = ctx.track(a, b, 1.0, 1.0, sigma, precision=form)
+    path = ctx.last_path()
+    assert np.array_equal(got[2], ref[2])
+    rms, mx = _errs(ref, got)
+    assert rms <= RMS_TOL and mx <= MAX_TOL, (path, rms, mx)

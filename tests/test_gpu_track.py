@@ -30,8 +30,8 @@ def test_reference_fixture(ctx, golden):
         assert np.sum(f32(a) == r) >= 0.998 * r.size
 
 
-SIGMAS_FULL = [4, 5, 6, 7, 8, 9, 10, 11]
-SIGMAS_SAMPLED = [12, 13, 14, 15, 16, 17, 18, 19, 20, 22]
+SIGMAS_FULL = [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11]          # boxes 4..44
+SIGMAS_SAMPLED = [12, 13, 14, 15, 16, 17, 18, 19, 20, 22, 23]  # 48..96 (no integer sigma gives 56)
 
 
 @pytest.mark.parametrize("sigma", SIGMAS_FULL)
@@ -58,6 +58,22 @@ def test_box_sizes_sampled(ctx, sigma):
     o = O.flct_pixels(a, b, 2.0, 0.5, sigma, pix)
     assert_parity((g[0].ravel()[pix], g[1].ravel()[pix], g[2].ravel()[pix]), o,
                   where=f"sigma {sigma} N {n}")
+
+
+def test_box_56_clamped_by_image_size(ctx):
+    """Box 56 has no integer sigma (nt jumps from 27 to 30 between sigma 13
+    and 14); sigma 14 on an image 85 rows high clamps nsize to 28
+    (flct.c:570-578)."""
+    rng = np.random.default_rng(56)
+    nx, ny = 85, 131
+    assert ctx.box_size(nx, ny, 14.0) == 56
+    a, b = smooth_pair(rng, nx, ny, shift=(0.9, -0.4), width=3.0)
+    g = ctx.track(a, b, 1.0, 1.0, 14.0)
+    i, j = np.mgrid[0:nx, 0:ny]
+    edge = (i < 2) | (i >= nx - 2) | (j < 2) | (j >= ny - 2)
+    pix = np.concatenate([np.flatnonzero(edge)[::5], rng.choice(nx * ny, 150, replace=False)])
+    o = O.flct_pixels(a, b, 1.0, 1.0, 14.0, pix)
+    assert_parity((g[0].ravel()[pix], g[1].ravel()[pix], g[2].ravel()[pix]), o, where="N 56")
 
 
 @pytest.mark.parametrize("thresh,absolute", [(0.3, False), (0.3, True), (0.05, False),
