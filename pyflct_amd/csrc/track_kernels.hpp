@@ -646,8 +646,10 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads), (TrackCfg<M, T>::k
             }
           }
           __syncthreads();
-          if (active && stid < kHiresWin * kHiresWin)
-            hires_cdata_entry(HiresWindow{hw, ixmax, iymax}, N, N, stid, hc);
+          // a slot has 2N threads, fewer than the 36 entries for boxes up to 16
+          if (active)
+            for (int t = stid; t < kHiresWin * kHiresWin; t += Cfg::kSlotThreads)
+              hires_cdata_entry(HiresWindow{hw, ixmax, iymax}, N, N, t, hc);
           __syncthreads();
           unsigned long long kbest = 0ull;
           int qbest = 0x7fffffff;
@@ -666,11 +668,14 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads), (TrackCfg<M, T>::k
           if (active && kbest == *hkey) atomicMin(hloc, (unsigned)qbest);
           __syncthreads();
           hq = (int)*hloc;
-          if (active && stid < 9) {
+          if (active) {
             // 3 x 3 neighbourhood of the grid maximum, where it is interior
-            const int ix = hq / kHiresN + stid / 3 - 1, iy = hq % kHiresN + stid % 3 - 1;
-            if (ix >= 0 && ix < kHiresN && iy >= 0 && iy < kHiresN)
-              nb[stid] = hires_value_cp(hc, N, N, ixmax, iymax, ix * kHiresN + iy);
+            // (9 entries; box 4 has 8 threads per slot)
+            for (int t = stid; t < 9; t += Cfg::kSlotThreads) {
+              const int ix = hq / kHiresN + t / 3 - 1, iy = hq % kHiresN + t % 3 - 1;
+              if (ix >= 0 && ix < kHiresN && iy >= 0 && iy < kHiresN)
+                nb[t] = hires_value_cp(hc, N, N, ixmax, iymax, ix * kHiresN + iy);
+            }
           }
         }
       }

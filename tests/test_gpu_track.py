@@ -74,6 +74,9 @@ def test_box_56_clamped_by_image_size(ctx):
     pix = np.concatenate([np.flatnonzero(edge)[::5], rng.choice(nx * ny, 150, replace=False)])
     o = O.flct_pixels(a, b, 1.0, 1.0, 14.0, pix)
     assert_parity((g[0].ravel()[pix], g[1].ravel()[pix], g[2].ravel()[pix]), o, where="N 56")
+    g = ctx.track(a, b, 1.0, 1.0, 14.0, hires=True)
+    o = O.flct_pixels(a, b, 1.0, 1.0, 14.0, pix, hires=True)
+    assert_parity((g[0].ravel()[pix], g[1].ravel()[pix], g[2].ravel()[pix]), o, where="N 56 -h")
 
 
 @pytest.mark.parametrize("thresh,absolute", [(0.3, False), (0.3, True), (0.05, False),
@@ -325,6 +328,29 @@ def test_hires_peak(ctx, sigma, kw):
     assert_parity(g, o, where=f"hires sigma={sigma} {kw}")
     plain = ctx.track(f1, f2, 1.0, 1.0, sigma, **kw)
     assert not np.array_equal(g[0], plain[0])
+
+
+HIRES_SIGMAS = [1, 2, 3, 7, 9, 11, 12, 13, 14, 16, 17, 18, 19, 20, 21, 22, 23]
+
+
+@pytest.mark.parametrize("sigma", HIRES_SIGMAS)
+def test_hires_every_box_size(ctx, sigma):
+    """-h in each FP64 box-size instantiation the cases above leave out
+    (boxes 4..96 but 56, which test_box_56_clamped_by_image_size covers), on
+    sampled pixels including the frame edges.  Boxes up to 16 have fewer
+    threads per slot (2N) than the 36 padded-block entries and the 9
+    neighbours the -h step fills."""
+    rng = np.random.default_rng(300 + sigma)
+    n = 2 * O.box_nsize(sigma, 4096, 4096)
+    nx, ny = int(1.6 * n) + 3, int(1.7 * n) + 4
+    a, b = smooth_pair(rng, nx, ny, shift=(0.35, -0.6), width=max(1.5, n / 16))
+    g = ctx.track(a, b, 1.0, 1.0, sigma, hires=True)
+    i, j = np.mgrid[0:nx, 0:ny]
+    edge = (i < 2) | (i >= nx - 2) | (j < 2) | (j >= ny - 2)
+    pix = np.concatenate([np.flatnonzero(edge)[::9], rng.choice(nx * ny, 60, replace=False)])
+    o = O.flct_pixels(a, b, 1.0, 1.0, sigma, pix, hires=True)
+    assert_parity((g[0].ravel()[pix], g[1].ravel()[pix], g[2].ravel()[pix]), o,
+                  where=f"hires sigma {sigma} N {n}")
 
 
 def test_hires_zero_images_and_other_forms(ctx):
