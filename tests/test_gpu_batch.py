@@ -119,3 +119,25 @@ def test_uneven_batch_forms_and_warp_residual(ctx):
         res = np.sqrt(np.mean((w[m, m] - b[m, m]) ** 2)) / np.sqrt(np.mean(b[m, m] ** 2))
         base = np.sqrt(np.mean((a[m, m] - b[m, m]) ** 2)) / np.sqrt(np.mean(b[m, m] ** 2))
         assert res < 0.5 * base, (a.shape, s, res, base)
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32", "split_bf16"])
+def test_batch_of_every_box_size(ctx, form):
+    """One graph launch holding a pair at every box size 4..96 (but 56), with
+    a relative threshold and a skip lattice on every other pair: each result
+    equals its single-pair call bit for bit."""
+    import oracle_lib as O
+    rng = np.random.default_rng(500)
+    items = []
+    for k, sigma in enumerate(range(1, 24)):
+        n = 2 * O.box_nsize(sigma, 4096, 4096)
+        a, b = smooth_pair(rng, int(1.6 * n) + 3 + k % 3, int(1.7 * n) + 1, shift=(-0.5, 0.7))
+        kw = {"precision": form}
+        if k % 2:
+            kw.update(thresh=0.2, skip=3, poff=1, interp=True)
+        items.append((a, b, 1.0, 1.0, float(sigma), kw))
+    got = ctx.track_batch(items)
+    for (a, b, dt, ds, s, kw), g in zip(items, got):
+        ref = ctx.track(a, b, dt, ds, s, **kw)
+        for k in range(3):
+            assert np.array_equal(g[k], ref[k]), (a.shape, s, kw, k)

@@ -394,3 +394,56 @@ def test_fp32_form_refines_unsettled_pixels(ctx):
     assert np.array_equal(g[2], ref[2])
     err = np.hypot(g[0] - ref[0], g[1] - ref[1])
     assert np.sqrt(np.mean(err ** 2)) <= 1e-4 and err.max() <= 1e-2, (err.max(), st["refined"])
+
+
+OPTION_SIGMAS = [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23]
+
+
+@pytest.mark.parametrize("sigma", OPTION_SIGMAS)
+def test_filter_and_bias_every_box_size(ctx, sigma):
+    """-k 0.6 -b at boxes 4..96 (but 56): FP64 against the oracle on sampled
+    pixels; FP32 and split-BF16 against FP64 (RMS <= 1e-4 px, max <= 5e-3
+    px, vm bit-exact)."""
+    rng = np.random.default_rng(400 + sigma)
+    n = 2 * O.box_nsize(sigma, 4096, 4096)
+    nx, ny = int(1.6 * n) + 5, int(1.7 * n) + 2
+    a, b = smooth_pair(rng, nx, ny, shift=(0.8, 0.45), width=max(1.5, n / 16))
+    kw = dict(kr=0.6, bias=True)
+    g = ctx.track(a, b, 1.0, 1.0, sigma, **kw)
+    i, j = np.mgrid[0:nx, 0:ny]
+    edge = (i < 2) | (i >= nx - 2) | (j < 2) | (j >= ny - 2)
+    pix = np.concatenate([np.flatnonzero(edge)[::11], rng.choice(nx * ny, 50, replace=False)])
+    o = O.flct_pixels(a, b, 1.0, 1.0, sigma, pix, **kw)
+    assert_parity((g[0].ravel()[pix], g[1].ravel()[pix], g[2].ravel()[pix]), o,
+                  where=f"-k -b sigma {sigma} N {n}")
+    for form in ("fp32", "split_bf16"):
+        f = ctx.track(a, b, 1.0, 1.0, sigma, precision=form, **kw)
+        assert np.array_equal(f[2], g[2]), form
+        e = np.hypot(f[0] - g[0], f[1] - g[1])
+        rms, mx = float(np.sqrt(np.mean(e ** 2))), float(e.max())
+        assert rms <= 1e-4 and mx <= 5e-3, (form, ctx.last_path(), rms, mx)
+
+
+@pytest.mark.parametrize("sigma", OPTION_SIGMAS)
+def test_threshold_and_skip_every_box_size(ctx, sigma):
+    """-t 0.25 -s 3p1q2i at boxes 4..96 (but 56): FP64 against the oracle
+    (whole image, fill included); FP32 and split-BF16 against FP64 with the
+    vm mask, the thresholded and the filled sets bit-exact."""
+    rng = np.random.default_rng(600 + sigma)
+    n = 2 * O.box_nsize(sigma, 4096, 4096)
+    nx, ny = int(1.6 * n) + 4, int(1.6 * n) + 9
+    a, b = smooth_pair(rng, nx, ny, shift=(0.3, -0.55), width=max(1.5, n / 16))
+    kw = dict(thresh=0.25, skip=3, poff=1, qoff=2, interp=True)
+    g = ctx.track(a, b, 1.0, 1.0, sigma, **kw)
+    o = O.flct(a, b, 1.0, 1.0, sigma, **kw)
+    assert_parity(g, o, where=f"-t -s sigma {sigma} N {n}")
+    for form in ("fp32", "split_bf16"):
+        f = ctx.track(a, b, 1.0, 1.0, sigma, precision=form, **kw)
+        assert np.array_equal(f[2], g[2]), form
+        z = g[2] == 0.0
+        assert np.array_equal(f[0][z], g[0][z]) and np.array_equal(f[1][z], g[1][z])
+        m = g[2] > 0.0
+        e = np.hypot(f[0][m] - g[0][m], f[1][m] - g[1][m])
+        if e.size:
+            rms, mx = float(np.sqrt(np.mean(e ** 2))), float(e.max())
+            assert rms <= 1e-4 and mx <= 5e-3, (form, ctx.last_path(), rms, mx)
