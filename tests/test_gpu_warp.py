@@ -156,3 +156,21 @@ def test_warp_nufft_far_shifts(ctx):
     assert np.max(np.abs(far - far_d)) < 1e-8 * scale
     near = ctx.warp(a, dx - 40 * 1000.0, dy + 48 * 300.0)
     assert np.max(np.abs(far - near)) < 1e-6 * scale
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (17, 4093), (4093, 23), (3001, 40), (16, 4096)])
+def test_warp_nufft_extreme_shapes(ctx, shape):
+    """The NUFFT at its limits (16 rows or columns, 4096 and odd lengths up to
+    it, where the spectrum transforms are Bluestein) against the direct sum."""
+    rng = np.random.default_rng(700 + shape[0])
+    a = rng.standard_normal(shape) * 5.0
+    dx = rng.standard_normal(shape) * 1.5
+    dy = rng.standard_normal(shape) * 1.5
+    wn = ctx.warp(a, dx, dy)
+    assert ctx.last_warp_method() == "nufft"
+    try:
+        ctx.set_warp_method("direct")
+        wd = ctx.warp(a, dx, dy)
+    finally:
+        ctx.set_warp_method("auto")
+    assert np.max(np.abs(wn - wd)) < 1e-11 * np.max(np.abs(a)), shape
