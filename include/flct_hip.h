@@ -156,9 +156,11 @@ const char *flct_hip_last_path(flct_hip_ctx *ctx);
  * on the given device inputs, times the whole call with HIP events, and
  * makes the fastest the context's path for that form and box size (until
  * the context is destroyed or tuned again).  Untuned, the split-BF16 form
- * runs the row-segment kernel for boxes up to 64 and the FP64 kernels above
- * (measured faster there).  sigma = 0 has one path: nothing to tune.  No
- * counterpart in the reference. */
+ * runs the FP32 kernels for boxes up to 20 and above 64 and the row-segment
+ * kernel from 24 to 64 (measured fastest there); its per-pixel MFMA kernels
+ * are eligible up to box 72, above which their error can exceed the form's
+ * contract.  sigma = 0 has one path: nothing to tune.  No counterpart in
+ * the reference. */
 int flct_hip_autotune(flct_hip_ctx *ctx, const double *d_f1, const double *d_f2, int nx, int ny,
                       const flct_hip_opts *opts);
 

@@ -136,7 +136,8 @@ extern "C" int flct_hip_create(flct_hip_ctx** out, int device) {
     }
   }
   // kernel choice of the split-BF16 form: "pixel" keeps every pixel on the
-  // per-pixel kernels (for comparisons); default is automatic
+  // per-pixel kernels where they meet the form's contract (boxes up to
+  // kBf16PixelMaxBox; for comparisons); default is automatic
   if (const char* k = std::getenv("FLCT_HIP_BF16_KERNEL"))
     c->bf16_kernel = (std::strcmp(k, "pixel") == 0) ? 1 : 0;
   *out = c;
@@ -258,6 +259,14 @@ static const char* path_name(int p) {
   }
 }
 
+// Largest box on which the per-pixel split-BF16 kernels keep the form's
+// contract (RMS <= 1e-4 px against FP64).  On smooth fields with broad
+// correlation peaks (random pairs about three boxes wide, smoothed over
+// box / 16 px, four seeds) the dense-DFT kernel's RMS error grows with the
+// box: at most 6.5e-5 px up to box 72, 8.8e-5 at 76 and 80, 1.2e-4 to
+// 1.5e-4 at 84, 92 and 96.
+constexpr int kBf16PixelMaxBox = 72;
+
 // candidate paths of a form and box size (the first is the untuned default
 // unless default_path says otherwise)
 static std::vector<int> path_candidates(int precision, int N, int nx, int ny, int bias,
@@ -267,7 +276,7 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
   if (precision == FLCT_HIP_SPLIT_BF16) {
     std::vector<int> v;
     if (seg_supported(N, nx, ny, bias)) v.push_back(PATH_BF16_SEG);
-    v.push_back(PATH_BF16_PIXEL);
+    if (N <= kBf16PixelMaxBox) v.push_back(PATH_BF16_PIXEL);
     v.push_back(PATH_FP32);
     v.push_back(PATH_FP64);
     return v;
@@ -285,7 +294,7 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
 static int default_path(int precision, int N, int nx, int ny, int bias, int bf16_kernel) {
   if (precision == FLCT_HIP_FP32) return PATH_FP32;
   if (precision == FLCT_HIP_SPLIT_BF16) {
-    if (bf16_kernel) return PATH_BF16_PIXEL;
+    if (bf16_kernel) return (N <= kBf16PixelMaxBox) ? PATH_BF16_PIXEL : PATH_FP32;
     if (N > 64 || N <= 20) return PATH_FP32;
     return seg_supported(N, nx, ny, bias) ? PATH_BF16_SEG : PATH_BF16_PIXEL;
   }

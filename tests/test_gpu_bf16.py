@@ -54,10 +54,11 @@ def test_bf16_matches_fp64(ctx, n, sigma, kw):
     assert mx <= MAX_TOL, (rms, mx)
 
 
-@pytest.mark.parametrize("n,sigma", [(256, 20.0), (256, 17.0), (192, 5.0)])
+@pytest.mark.parametrize("n,sigma", [(256, 16.0), (256, 17.0), (192, 5.0)])
 def test_bf16_dense_kernel_large_boxes(monkeypatch, n, sigma):
-    """The per-pixel dense-DFT BF16 kernel (boxes up to 96), which the
-    untuned path rule does not pick above 64, against FP64."""
+    """The per-pixel dense-DFT BF16 kernel at boxes 68, 72 and 20 (it is
+    eligible up to 72), which the untuned path rule does not pick above 64,
+    against FP64."""
     from pyflct_amd import api
     monkeypatch.setenv("FLCT_HIP_BF16_KERNEL", "pixel")
     c = api.Context(0)
@@ -117,6 +118,7 @@ SEG_SHAPES = [
     (60, 77, 2.0, {}),                            # N = 8
     (70, 81, 3.0, {"kr": 0.6}),                   # N = 12 with the filter
     (100, 120, 4.0, {}),                          # N = 16
+    (90, 101, 5.0, {"kr": 0.45}),                 # N = 20 with the filter
     (110, 97, 7.0, {"thresh": 0.25}),             # N = 28, relative threshold
     (140, 150, 8.0, {}),                          # N = 32
 ]
@@ -173,7 +175,8 @@ def test_bf16_segment_matches_pixel_kernel(monkeypatch):
 def test_set_path_rules(ctx):
     from pyflct_amd._lib import FlctHipError
     for args in (("fp64", 64, "bf16-segment"), ("fp32", 64, "bf16-pixel"), ("split_bf16", 66, "fp64-fft"),
-                 ("split_bf16", 64, "no-such-path"), ("split_bf16", 84, "bf16-segment")):
+                 ("split_bf16", 64, "no-such-path"), ("split_bf16", 84, "bf16-segment"),
+                 ("split_bf16", 76, "bf16-pixel")):
         with pytest.raises(FlctHipError):
             ctx.set_path(*args)
     f1, f2, _ = synth.cfg4(n=192)
@@ -263,3 +266,27 @@ def test_forms_every_box_size(ctx, form, nbox):
     assert np.array_equal(got[2], ref[2])
     rms, mx = _errs(ref, got)
     assert rms <= RMS_TOL and mx <= MAX_TOL, (path, rms, mx)
+
+
+def test_pixel_kernels_every_box_size(monkeypatch):
+    """The per-pixel split-BF16 kernels (FLCT_HIP_BF16_KERNEL=pixel: the
+    dense-DFT kernel and the radix-2 kernel for boxes 36..64) at every box
+    size up to 72 against FP64; above 72, where their error can exceed the
+    form's contract, the request runs the FP32 kernels."""
+    from pyflct_amd import api
+    from conftest import smooth_pair
+    monkeypatch.setenv("FLCT_HIP_BF16_KERNEL", "pixel")
+    c = api.Context(0)
+    try:
+        for nbox in BOXES:
+            sigma, nx, ny = _box_case(nbox)
+            rng = np.random.default_rng(900 + nbox)
+            a, b = smooth_pair(rng, nx, ny, shift=(-0.7, 0.4), width=max(1.5, nbox / 16))
+            ref = c.track(a, b, 1.0, 1.0, sigma)
+            got = c.track(a, b, 1.0, 1.0, sigma, precision="split_bf16")
+            assert c.last_path() == ("bf16-pixel" if nbox <= 72 else "fp32-fft"), (nbox, c.last_path())
+            assert np.array_equal(got[2], ref[2]), nbox
+            rms, mx = _errs(ref, got)
+            assert rms <= RMS_TOL and mx <= MAX_TOL, (nbox, rms, mx)
+    finally:
+        c.close()
