@@ -447,3 +447,22 @@ def test_threshold_and_skip_every_box_size(ctx, sigma):
         if e.size:
             rms, mx = float(np.sqrt(np.mean(e ** 2))), float(e.max())
             assert rms <= 1e-4 and mx <= 5e-3, (form, ctx.last_path(), rms, mx)
+
+
+def test_large_image_8192(ctx):
+    """8192 x 8192 (67 M pixels, box 64): FP64 against the oracle on sampled
+    pixels including the four frame edges, split-BF16 against FP64."""
+    rng = np.random.default_rng(8192)
+    n = 8192
+    a, b = smooth_pair(rng, n, n, shift=(0.45, -0.3), width=3.0)
+    g = ctx.track(a, b, 1.0, 1.0, 15.0)
+    assert ctx.last_stats()["tracked"] == n * n
+    pix = np.concatenate([rng.choice(n * n, 40, replace=False),
+                          [0, n - 1, n * (n - 1), n * n - 1, n * (n // 2), n * (n // 2) + n - 1]])
+    o = O.flct_pixels(a, b, 1.0, 1.0, 15.0, pix)
+    assert_parity((g[0].ravel()[pix], g[1].ravel()[pix], g[2].ravel()[pix]), o, where="8192^2")
+    s = ctx.track(a, b, 1.0, 1.0, 15.0, precision="split_bf16")
+    assert np.array_equal(s[2], g[2])
+    e = np.hypot(s[0] - g[0], s[1] - g[1])
+    rms = float(np.sqrt(np.mean(e ** 2)))
+    assert rms <= 1e-4 and float(e.max()) <= 5e-3, (rms, float(e.max()))
