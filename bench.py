@@ -113,8 +113,8 @@ def main():
     ap.add_argument("--pairs", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU runs are not supported "
-              "(the tracker shards by row strips, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: runs on one GPU only; multi-GPU runs are not supported "
+              "(across GPUs the tracker would take row strips of the image, one per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from pyflct_amd import api, synth

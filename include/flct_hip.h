@@ -25,7 +25,7 @@ extern "C" {
 #define FLCT_HIP_E_PQ_RANGE 3      /* flct.c:466-472  "abs(p) or abs(q) >= skip" */
 #define FLCT_HIP_E_SKIP_TOO_BIG 4  /* flct.c:559-563  "skip = %d is too big" */
 #define FLCT_HIP_E_BOX 5           /* flct.c:575-579  "illegal box size" */
-#define FLCT_HIP_E_UNSUPPORTED 6   /* a request this build does not handle */
+#define FLCT_HIP_E_UNSUPPORTED 6   /* a request the library does not handle */
 #define FLCT_HIP_E_NODEVICE 7      /* no usable gfx950 device */
 #define FLCT_HIP_E_ALLOC 8         /* device allocation failed */
 #define FLCT_HIP_E_HIP 9           /* a HIP runtime call failed */

@@ -54,7 +54,7 @@ def test_warp_rejects_bad_shift_arrays(tmp_path):
     vcimage.write(img, np.zeros((10, 12)))
     vcimage.write(sh, np.zeros((3, 4)), np.zeros((3, 4)))
     rc, out = run(WARP, str(img), str(sh), str(tmp_path / "o.dat"))
-    # the device is opened after the shapes are known only on a GPU box;
+    # the device is opened after the shapes are known only on a machine with a GPU;
     # either way the run must fail
     assert rc == 1
 
