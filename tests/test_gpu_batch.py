@@ -141,3 +141,24 @@ def test_batch_of_every_box_size(ctx, form):
         ref = ctx.track(a, b, dt, ds, s, **kw)
         for k in range(3):
             assert np.array_equal(g[k], ref[k]), (a.shape, s, kw, k)
+
+
+def test_batch_mixed_hires_bias_filter_forms(ctx):
+    """-h, -b and -k pairs of different forms in one graph launch equal their
+    single-pair calls bit for bit (-h runs the FP64 kernels in every form)."""
+    rng = np.random.default_rng(501)
+    specs = [(96, 110, 5.0, {"hires": True}),
+             (150, 140, 15.0, {"hires": True, "precision": "split_bf16"}),
+             (120, 128, 6.0, {"bias": True, "precision": "fp32"}),
+             (200, 210, 12.0, {"bias": True, "kr": 0.5, "precision": "split_bf16"}),
+             (64, 70, 2.0, {"hires": True, "kr": 0.7}),
+             (256, 240, 20.0, {"kr": 0.4, "precision": "split_bf16"})]
+    items = []
+    for nx, ny, sigma, kw in specs:
+        a, b = smooth_pair(rng, nx, ny, shift=(0.35, 0.8), width=2.0)
+        items.append((a, b, 1.0, 1.0, sigma, kw))
+    got = ctx.track_batch(items)
+    for (a, b, dt, ds, s, kw), g in zip(items, got):
+        ref = ctx.track(a, b, dt, ds, s, **kw)
+        for k in range(3):
+            assert np.array_equal(g[k], ref[k]), (a.shape, s, kw, k)
