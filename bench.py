@@ -72,6 +72,15 @@ def r2_mfma_flops(n, bias=False):
     return 88 * 3 * MFMA_FLOPS * 2 * (2 if bias else 1)
 
 
+def form_roofline(form, r, nbox):
+    """Algorithmic FFT-form flops (w_fft) per tracked pixel over the tracking
+    kernel's HIP-event time, against the form's vector peak."""
+    peak = PEAK_FP64_TFLOPS if form == "fp64" else PEAK_FP32_TFLOPS
+    achieved = w_fft(nbox) * r["tracked"] / (r["main_ms"] * 1e-3) / 1e12
+    return {"achieved_TFLOPs": round(achieved, 3), "peak_TFLOPs": peak, "frac": round(achieved / peak, 4),
+            "flops_per_px": w_fft(nbox)}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -292,7 +301,8 @@ def main():
                       "kernel_ms": round(r["kernel_ms"], 3), "main_ms": round(r["main_ms"], 3),
                       "refine_ms": round(r["refine_ms"], 3), "refined_px": r["refined"],
                       "rms_vel_err_vs_fp64": r["rms_vel_err_vs_fp64"],
-                      "max_vel_err_vs_fp64": r["max_vel_err_vs_fp64"]}
+                      "max_vel_err_vs_fp64": r["max_vel_err_vs_fp64"],
+                      **({"roofline_fft": form_roofline(f, r, nbox)} if f in ("fp64", "fp32") else {})}
                   for f, r in results.items()},
         "fp64_vs_oracle_max_err_px": parity,
         "end_to_end": end_to_end,
