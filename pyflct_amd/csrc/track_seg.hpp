@@ -423,8 +423,10 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           Vm[X].re = er - orr;
           Vm[X].im = ei - oi;
         });
+        // no compiler barrier inside the kt loops: the next tile's loads may
+        // be issued ahead of this tile's MFMAs (1.5 % faster than with one,
+        // same registers and spills)
         static_for<TL>([&](auto kt) {
-          asm volatile("" ::: "memory");
           const int k = 16 * kt + n;
           // stage-2 tile of image X: S(l), S(l + Nh) for the lane's rows
           auto stage2 = [&](auto X, CF4& sp, CF4& sm) {
@@ -539,7 +541,6 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       // ---------------- inverse 1: A = Q_q (row k, contraction over l)
       unsigned Uh[2][2][2][4], Ul[2][2][2][4];  // [kt][NT][q][t']
       static_for<TL>([&](auto kt) {
-        asm volatile("" ::: "memory");
         static_for<2>([&](auto q) {
           u32x4 rh, rl, ih, il;
           static_for<2>([&](auto LT) {
