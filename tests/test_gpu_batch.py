@@ -162,3 +162,23 @@ def test_batch_mixed_hires_bias_filter_forms(ctx):
         ref = ctx.track(a, b, dt, ds, s, **kw)
         for k in range(3):
             assert np.array_equal(g[k], ref[k]), (a.shape, s, kw, k)
+
+
+def test_batch_of_300_pairs(ctx):
+    """A graph of 300 small pairs (sizes 40..71, sigma 2..6, every form):
+    results equal single-pair calls on every tenth pair, the graph replays."""
+    rng = np.random.default_rng(502)
+    items = []
+    forms = ("fp64", "fp32", "split_bf16")
+    for k in range(300):
+        nx, ny = int(rng.integers(40, 72)), int(rng.integers(40, 72))
+        a, b = smooth_pair(rng, nx, ny, shift=(0.2, -0.4), width=1.8)
+        items.append((a, b, 1.0, 1.0, float(rng.integers(2, 7)), {"precision": forms[k % 3]}))
+    got = ctx.track_batch(items)
+    again = ctx.track_batch(items)
+    for k in range(0, 300, 10):
+        a, b, dt, ds, s, kw = items[k]
+        ref = ctx.track(a, b, dt, ds, s, **kw)
+        for c in range(3):
+            assert np.array_equal(got[k][c], ref[c]), (k, a.shape, s, kw, c)
+            assert np.array_equal(again[k][c], ref[c]), (k, a.shape, s, kw, c)
