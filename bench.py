@@ -120,6 +120,8 @@ def main():
                          "of sizes 256..2048 and sigma 4..20 in one graph launch; configs: the "
                          "128^2, 512^2 and 1024^2 cases with their rates and errors")
     ap.add_argument("--pairs", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="uneven-batch: time the untuned path rule only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only; multi-GPU runs are not supported "
@@ -360,6 +362,39 @@ def uneven_batch(args):
         res[form] = {"ms": float(np.mean(ms)), "px_per_s": tracked / (np.mean(ms) * 1e-3),
                      "tracked": tracked, "refined": sum(x["refined"] for x in st)}
         log(f"bench: {form}: {np.mean(ms):.1f} ms per batch, {res[form]['px_per_s'] / 1e6:.1f} Mpx/s")
+    # the autotuner: each (form, box size) is tuned on the largest pair of
+    # that box size, then the batch is timed again with the tuned paths (a
+    # tuning changes the context's paths, so the batch graph is captured anew)
+    tuned = {}
+    if not args.no_autotune:
+        by_box = {}
+        for n, sigma, d in pairs:
+            nb = api.Context.box_size(n, n, sigma)
+            if nb not in by_box or n > by_box[nb][0]:
+                by_box[nb] = (n, sigma, d)
+        for form in ("fp32", "split_bf16"):
+            t0 = time.perf_counter()
+            paths = {}
+            for nb, (n, sigma, d) in sorted(by_box.items()):
+                ctx.autotune(d["f1"], d["f2"], n, n, ctx.make_opts(1.0, 1.0, sigma, precision=form))
+                paths[nb] = ctx.last_path()
+            tune_s = time.perf_counter() - t0
+            desc = [(d["f1"], d["f2"], d[f"{form}_vx"], d[f"{form}_vy"], d[f"{form}_vm"], n, n,
+                     ctx.make_opts(1.0, 1.0, sigma, precision=form)) for n, sigma, d in pairs]
+            for _ in range(max(1, args.warmup)):
+                ctx.track_batch_device(desc)
+            ms = []
+            for _ in range(args.steps):
+                st = ctx.track_batch_device(desc)
+                ms.append(ctx.last_stats()["kernel_ms"])
+            tracked = sum(x["tracked"] for x in st)
+            tuned[form] = {"ms": float(np.mean(ms)), "px_per_s": tracked / (np.mean(ms) * 1e-3),
+                           "refined": sum(x["refined"] for x in st), "tune_s": tune_s,
+                           "paths": {str(k): v for k, v in paths.items()}}
+            log(f"bench: {form} tuned ({tune_s:.1f} s of tuning): {np.mean(ms):.1f} ms per batch, "
+                f"{tuned[form]['px_per_s'] / 1e6:.1f} Mpx/s; paths {paths}")
+    # with the autotuner on, the FP32 and split-BF16 velocities compared
+    # below are the tuned batches'
     # the warp of every image 1 by its split-BF16 velocities, timed (device
     # time per call, summed over the pairs), then the same for one 2048^2
     # pair with the reference's direct per-pixel sum
@@ -400,7 +435,7 @@ def uneven_batch(args):
     for n, sigma, d in pairs:
         for p_ in d.values():
             ctx.free(p_)
-    hr = res["split_bf16"]
+    hr = tuned.get("split_bf16", res["split_bf16"])
     out = {
         "metric": "uneven batch tracked px/s (64 pairs, 256..2048, sigma 4..20), split-BF16",
         "value": round(hr["px_per_s"], 1),
@@ -414,6 +449,9 @@ def uneven_batch(args):
                    "total_px": total_px},
         "forms": {f: {"px_per_s": round(r["px_per_s"], 1), "ms_per_batch": round(r["ms"], 3),
                       "refined_px": r["refined"]} for f, r in res.items()},
+        "autotuned": {f: {"px_per_s": round(r["px_per_s"], 1), "ms_per_batch": round(r["ms"], 3),
+                          "refined_px": r["refined"], "tuning_s": round(r["tune_s"], 2),
+                          "path_by_box": r["paths"]} for f, r in tuned.items()},
         "rms_vel_err_vs_fp64": {f: float(np.sqrt(sq[f] / total_px)) for f in sq},
         "max_vel_err_vs_fp64": mx,
         "warp_residual_rms": {"split_bf16_velocities": float(np.sqrt(np.mean(np.square(rres)))),
