@@ -45,8 +45,9 @@ typedef struct flct_hip_ctx flct_hip_ctx;
 typedef struct flct_hip_opts {
   double deltat;     /* time between images */
   double deltas;     /* length unit of one pixel */
-  double sigma;      /* Gaussian window width in pixels; 0 = one whole-image
-                        correlation (flct.c:361-365), < 0 is an illegal box */
+  double sigma;      /* Gaussian window width in pixels; any value not > 0
+                        (0, negative) = one whole-image correlation
+                        (flct.c:357-365, 567-586) */
   double thresh;     /* -t value (0: off) */
   int thresh_abs;    /* -t value carried an 'a' suffix: absolute units */
   int skip;          /* -s N (0: off) */

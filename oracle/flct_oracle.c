@@ -23,8 +23,11 @@
  *     integer-shift warp == roll, uniform warp == shift_frac2d).
  *   - the -h peak (oracle_peak_hires) on sampled quadratics, whose cubic
  *     convolution is exact, and its all-zero and edge rules;
- *   - the -b window-offset bias correction has no reference counterpart:
- *     parity unpinned (only invariants check it).
+ *   - the -b window-offset bias correction (no reference counterpart) by a
+ *     direct-sum restatement of its definition on boxes 8 and 12, by exact
+ *     cases (d = 0 is the plain pass bit for bit; a zero-sum integer blob
+ *     moved by an integer s, given d = s, returns s; swapping the images
+ *     negates the velocity) and by four planted mistakes.
  *
  * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).
  *
@@ -624,24 +627,19 @@ static double gauss_off(const track_ctx_t *t, int di, int dj) {
   return t->gtab[(di + nb - 1) * (2 * nb - 1) + (dj + nb - 1)];
 }
 
-/* one pixel: box clamp (773-808), means (815-827), window (856-870),
- * cross_cor (876), velocity (901-905).  With bias != 0 the windows are
- * moved to -d/2 (image 1) and +d/2 (image 2) about the pixel using the
- * first estimate d, and the correlation is repeated once (window-offset
- * correction, central-difference form; no reference counterpart). */
-static void track_pixel(track_ctx_t *t, const double *f1, const double *f2, int i,
-                        int j, const oracle_opts_t *o, double *vxx, double *vyy) {
+/* box of pixel (i, j) and its means: clamp flct.c:773-808, means
+ * flct.c:815-827 (summed row-major over the clamped box) */
+static void pixel_box(const track_ctx_t *t, const double *f1, const double *f2, int i, int j,
+                      int *imin_out, int *jmin_out, double *f1bar_out, double *f2bar_out) {
   int nx = t->nx, ny = t->ny, nsize = t->nsize, ii, jj;
   int imin0, imax0, imin, jmin0, jmax0, jmin, isize, jsize;
-  double f1bar = 0., f2bar = 0., shiftx, shifty, deltinv = 1. / o->deltat;
+  double f1bar = 0., f2bar = 0.;
   imin0 = (0 > (i - (nsize - 1))) ? 0 : i - (nsize - 1);
   imax0 = ((nx - 1) < (i + nsize)) ? nx - 1 : i + nsize;
   imin = (imax0 == nx - 1) ? nx - 1 - (2 * nsize - 1) : imin0;
   jmin0 = (0 > (j - (nsize - 1))) ? 0 : j - (nsize - 1);
   jmax0 = ((ny - 1) < (j + nsize)) ? ny - 1 : j + nsize;
   jmin = (jmax0 == ny - 1) ? ny - 1 - (2 * nsize - 1) : jmin0;
-  (void)imax0;
-  (void)jmax0;
   isize = jsize = 2 * nsize;
   for (ii = 0; ii < isize; ii++)
     for (jj = 0; jj < jsize; jj++) {
@@ -653,6 +651,82 @@ static void track_pixel(track_ctx_t *t, const double *f1, const double *f2, int 
 #ifdef PLANT_NO_MEAN
   f1bar = f2bar = 0.;
 #endif
+  *imin_out = imin;
+  *jmin_out = jmin;
+  *f1bar_out = f1bar;
+  *f2bar_out = f2bar;
+}
+
+/* -b second pass (no reference counterpart; window-offset correction,
+ * central-difference form): with the first estimate d = (dsx, dsy) in
+ * surface units (rows, columns), image 1 is windowed by the Gaussian
+ * centred at pixel - d/2 and image 2 by the one centred at pixel + d/2,
+ * over the same box and with the same box means, and cross_cor runs again.
+ * d = 0 reproduces the plain pass bit for bit (same exp arguments). */
+static void bias_pass(track_ctx_t *t, const double *f1, const double *f2, int i, int j,
+                      int imin, int jmin, double f1bar, double f2bar, int hires, double dsx,
+                      double dsy, double *shiftx, double *shifty) {
+  int ny = t->ny, isize = t->nbox, jsize = t->nbox, ii, jj;
+  double cx = 0.5 * dsx, cy = 0.5 * dsy;
+#ifdef PLANT_BIAS_FULL_D
+  cx = dsx;
+  cy = dsy;
+#endif
+#ifdef PLANT_BIAS_REMEAN
+  {
+    double s1 = 0., s2 = 0., w1s = 0., w2s = 0.;
+    for (ii = 0; ii < isize; ii++)
+      for (jj = 0; jj < jsize; jj++) {
+        double dx = (double)(ii + imin - i), dy = (double)(jj + jmin - j);
+        double a1 = t->sigminv * (dx + cx), a2 = t->sigminv * (dx - cx);
+        double b1 = t->sigminv * (dy + cy), b2 = t->sigminv * (dy - cy);
+        double w1 = exp(-a1 * a1 - b1 * b1), w2 = exp(-a2 * a2 - b2 * b2);
+        s1 += w1 * f1[(ii + imin) * ny + (jj + jmin)];
+        s2 += w2 * f2[(ii + imin) * ny + (jj + jmin)];
+        w1s += w1;
+        w2s += w2;
+      }
+    f1bar = s1 / w1s;
+    f2bar = s2 / w2s;
+  }
+#endif
+  for (ii = 0; ii < isize; ii++) {
+    double dx = (double)(ii + imin - i);
+    double a1 = t->sigminv * (dx + cx), a2 = t->sigminv * (dx - cx);
+#ifdef PLANT_BIAS_SIGN
+    a1 = t->sigminv * (dx - cx);
+    a2 = t->sigminv * (dx + cx);
+#endif
+#ifdef PLANT_BIAS_ONE_IMAGE
+    a1 = t->sigminv * dx;
+#endif
+    for (jj = 0; jj < jsize; jj++) {
+      double dy = (double)(jj + jmin - j);
+      double b1 = t->sigminv * (dy + cy), b2 = t->sigminv * (dy - cy);
+#ifdef PLANT_BIAS_SIGN
+      b1 = t->sigminv * (dy - cy);
+      b2 = t->sigminv * (dy + cy);
+#endif
+#ifdef PLANT_BIAS_ONE_IMAGE
+      b1 = t->sigminv * dy;
+#endif
+      double w1 = exp(-a1 * a1 - b1 * b1), w2 = exp(-a2 * a2 - b2 * b2);
+      t->g1[ii * jsize + jj] = w1 * (f1[(ii + imin) * ny + (jj + jmin)] - f1bar);
+      t->g2[ii * jsize + jj] = w2 * (f2[(ii + imin) * ny + (jj + jmin)] - f2bar);
+    }
+  }
+  xcor_surface(&t->ws, t->g1, t->g2);
+  cc_peak(hires, t->ws.shifted, isize, jsize, shiftx, shifty);
+}
+
+/* one pixel: box clamp (773-808), means (815-827), window (856-870),
+ * cross_cor (876), velocity (901-905).  With bias != 0 the plain estimate
+ * d feeds bias_pass once and its result replaces d. */
+static void track_pixel(track_ctx_t *t, const double *f1, const double *f2, int i,
+                        int j, const oracle_opts_t *o, double *vxx, double *vyy) {
+  int ny = t->ny, ii, jj, imin, jmin, isize = t->nbox, jsize = t->nbox;
+  double f1bar, f2bar, shiftx, shifty, deltinv = 1. / o->deltat;
+  pixel_box(t, f1, f2, i, j, &imin, &jmin, &f1bar, &f2bar);
   for (ii = 0; ii < isize; ii++)
     for (jj = 0; jj < jsize; jj++) {
       double g = gauss_off(t, ii + imin - i, jj + jmin - j);
@@ -661,22 +735,9 @@ static void track_pixel(track_ctx_t *t, const double *f1, const double *f2, int 
     }
   xcor_surface(&t->ws, t->g1, t->g2);
   cc_peak(o->hires, t->ws.shifted, isize, jsize, &shiftx, &shifty);
-  if (o->bias) {
-    double cx = 0.5 * shiftx, cy = 0.5 * shifty;
-    for (ii = 0; ii < isize; ii++) {
-      double dx = (double)(ii + imin - i);
-      double a1 = t->sigminv * (dx + cx), a2 = t->sigminv * (dx - cx);
-      for (jj = 0; jj < jsize; jj++) {
-        double dy = (double)(jj + jmin - j);
-        double b1 = t->sigminv * (dy + cy), b2 = t->sigminv * (dy - cy);
-        double w1 = exp(-a1 * a1 - b1 * b1), w2 = exp(-a2 * a2 - b2 * b2);
-        t->g1[ii * jsize + jj] = w1 * (f1[(ii + imin) * ny + (jj + jmin)] - f1bar);
-        t->g2[ii * jsize + jj] = w2 * (f2[(ii + imin) * ny + (jj + jmin)] - f2bar);
-      }
-    }
-    xcor_surface(&t->ws, t->g1, t->g2);
-    cc_peak(o->hires, t->ws.shifted, isize, jsize, &shiftx, &shifty);
-  }
+  if (o->bias)
+    bias_pass(t, f1, f2, i, j, imin, jmin, f1bar, f2bar, o->hires, shiftx, shifty, &shiftx,
+              &shifty);
   /* transp = 1 (flct.c:901-905) */
 #ifdef PLANT_NO_SWAP
   *vxx = shiftx * deltinv * o->deltas;
@@ -815,7 +876,8 @@ int oracle_flct(const double *f1, const double *f2, int nx, int ny, double delta
   o.filter = filter; o.kr = kr; o.bias = bias; o.hires = hires;
   rc = check_opts(&o, nx, ny, &poff, &qoff, &skipon);
   if (rc) return rc;
-  if (sigma == 0.) return flct_whole(f1, f2, nx, ny, &o, vx, vy, vm);
+  /* sigmaeq0 for any sigma that is not > 0 (flct.c:357-365) */
+  if (!(sigma > 0.)) return flct_whole(f1, f2, nx, ny, &o, vx, vy, vm);
   rc = track_ctx_init(&t, nx, ny, &o);
   if (rc) return rc;
   thresh = resolve_thresh(f1, f2, nx * ny, &o);
@@ -912,7 +974,7 @@ int oracle_flct_pixels(const double *f1, const double *f2, int nx, int ny,
   o.filter = filter; o.kr = kr; o.bias = bias; o.hires = hires;
   rc = check_opts(&o, nx, ny, &poff, &qoff, &skipon);
   if (rc) return rc;
-  if (sigma == 0.) return OR_E_UNSUPPORTED;  /* one whole-image velocity: oracle_flct */
+  if (!(sigma > 0.)) return OR_E_UNSUPPORTED;  /* one whole-image velocity: oracle_flct */
   rc = track_ctx_init(&t, nx, ny, &o);
   if (rc) return rc;
   thresh = resolve_thresh(f1, f2, nx * ny, &o);
@@ -930,6 +992,29 @@ int oracle_flct_pixels(const double *f1, const double *f2, int nx, int ny,
     vy[p] = vyy;
     vm[p] = vmask;
   }
+  track_ctx_free(&t);
+  return OR_OK;
+}
+
+/* Test hook of the -b pins: the second pass alone, for pixel (i, j), with a
+ * given offset d = (dsx, dsy) in surface units (rows, columns) instead of
+ * the plain estimate.  Returns the surface shifts (before the transp swap
+ * of flct.c:901-905).  d = 0 is the plain pass. */
+int oracle_bias_pass(const double *f1, const double *f2, int nx, int ny, double sigma,
+                     int filter, double kr, int hires, int i, int j, double dsx, double dsy,
+                     double *shiftx, double *shifty) {
+  oracle_opts_t o;
+  track_ctx_t t;
+  int imin, jmin, rc;
+  double f1bar, f2bar;
+  memset(&o, 0, sizeof(o));
+  o.deltat = 1.; o.deltas = 1.; o.sigma = sigma; o.filter = filter; o.kr = kr;
+  o.hires = hires;
+  if (!(sigma > 0.) || i < 0 || i >= nx || j < 0 || j >= ny) return OR_E_ARG;
+  rc = track_ctx_init(&t, nx, ny, &o);
+  if (rc) return rc;
+  pixel_box(&t, f1, f2, i, j, &imin, &jmin, &f1bar, &f2bar);
+  bias_pass(&t, f1, f2, i, j, imin, jmin, f1bar, f2bar, hires, dsx, dsy, shiftx, shifty);
   track_ctx_free(&t);
   return OR_OK;
 }

@@ -143,14 +143,14 @@ class Context:
     def track(self, f1, f2, deltat, deltas, sigma, **kw):
         """Track on host arrays; returns (vx, vy, vm) float64 arrays.
 
-        With ``sigma = 0`` the two whole images are correlated once and the
-        arrays are 1 x 1 (flct.c:581-586).
+        With ``sigma <= 0`` the two whole images are correlated once and the
+        arrays are 1 x 1 (flct.c:357-365, 581-586).
         """
         f1, f2 = f64c(f1), f64c(f2)
         if f1.ndim != 2 or f1.shape != f2.shape:
             raise ValueError("f1 and f2 must be 2-D arrays of the same shape")
         nx, ny = f1.shape
-        shape = (1, 1) if sigma == 0 else (nx, ny)
+        shape = (nx, ny) if sigma > 0 else (1, 1)
         vx, vy, vm = (np.empty(shape) for _ in range(3))
         o = self.make_opts(deltat, deltas, sigma, **kw)
         check(_lib.lib().flct_hip_track(self._h, ptr(f1), ptr(f2), nx, ny, ctypes.byref(o),

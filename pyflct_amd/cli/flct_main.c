@@ -24,7 +24,7 @@ static void usage(const char *prog) {
   printf("  ifile   two images (vcimage format)\n");
   printf("  ofile   vx, vy, vm (vm: 1 computed, 0 not computed, 0.5 interpolated)\n");
   printf("  deltat  time between the images; deltas  length of one pixel\n");
-  printf("  sigma   width in pixels of the Gaussian window (> 0)\n");
+  printf("  sigma   width in pixels of the Gaussian window; 0 (or less) correlates the whole images once\n");
   printf("  -t thr  skip pixels whose mean |image| is below thr; 0 < thr < 1 is\n");
   printf("          relative to the largest |pixel| unless followed by 'a'\n");
   printf("  -k kr   Gaussian low-pass of the sub-images, kr in units of the Nyquist wavenumber\n");
@@ -124,7 +124,7 @@ int main(int argc, char *argv[]) {
     printf("flct: %s\n", flct_hip_strerror(rc));
     exit(1);
   }
-  if (!quiet && o.sigma != 0.) printf("flct: nominal sliding box size = %d\n", nbox);
+  if (!quiet && o.sigma > 0.) printf("flct: nominal sliding box size = %d\n", nbox);
   vx = (double *)malloc(sizeof(double) * (size_t)nx * ny);
   vy = (double *)malloc(sizeof(double) * (size_t)nx * ny);
   vm = (double *)malloc(sizeof(double) * (size_t)nx * ny);
@@ -148,7 +148,7 @@ int main(int argc, char *argv[]) {
     flct_hip_last_stats(ctx, &tracked, &kms, &tms);
     printf("flct: tracked %lld pixels in %.3f ms on the GPU\n", tracked, tms);
   }
-  if (o.sigma == 0.) {
+  if (!(o.sigma > 0.)) {
     /* one velocity for the whole image, written as 1 x 1 arrays (flct.c:581-586) */
     if (!quiet) printf("\nflct: vx = %g vy = %g \n", vx[0], vy[0]);
     nx = 1;

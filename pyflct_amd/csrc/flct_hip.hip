@@ -204,7 +204,6 @@ struct Resolved {
 // argument rules of the reference CLI (flct.c:459-472, 537-538, 559-563,
 // 567-580)
 int resolve(const flct_hip_opts* o, int nx, int ny, Resolved* r) {
-  if (o->sigma != o->sigma) return FLCT_HIP_E_ARG;
   int p = o->poff, q = o->qoff;
   r->skipon = 0;
   if (o->skip != 0 || p != 0 || q != 0) {
@@ -217,7 +216,10 @@ int resolve(const flct_hip_opts* o, int nx, int ny, Resolved* r) {
   if ((o->skip >= nx) || (o->skip >= ny)) return FLCT_HIP_E_SKIP_TOO_BIG;
   r->poff = p;
   r->qoff = q;
-  if (o->sigma == 0.) {  // flct.c:361-365, 567-586
+  // any sigma that is not > 0 (zero, negative, NaN) correlates the whole
+  // image once: flct.c:357-365 sets sigmaeq0 and skips the box rules
+  // (flct.c:567-586)
+  if (!(o->sigma > 0.)) {
     r->whole = 1;
     return FLCT_HIP_OK;
   }
@@ -803,7 +805,7 @@ extern "C" int flct_hip_track(flct_hip_ctx* c, const double* f1, const double* f
   HIPCHK(hipSetDevice(c->device));
   const size_t bytes = (size_t)nx * ny * sizeof(double);
   int rc;
-  if (o->sigma == 0.) {  // one velocity for the whole image: vx[0], vy[0], vm[0]
+  if (!(o->sigma > 0.)) {  // one velocity for the whole image: vx[0], vy[0], vm[0]
     Resolved r0;
     if ((rc = resolve(o, nx, ny, &r0))) return rc;
     if (o->deltat == 0.) return FLCT_HIP_E_ARG;

@@ -74,6 +74,7 @@ def _bind(path):
     L.oracle_warp_frac2d_rows.argtypes = [_dp, _dp, _dp, _dp, i, i, i, i, i]
     L.oracle_shift_frac2d.argtypes = [_dp, d, d, _dp, i, i, i]
     L.oracle_resolve_thresh.argtypes = [_dp, _dp, i, d, i, _dp]
+    L.oracle_bias_pass.argtypes = [_dp, _dp, i, i, d, i, d, i, i, i, d, d, _dp, _dp]
     return L
 
 
@@ -84,10 +85,10 @@ def _f64(a):
 def flct(f1, f2, deltat, deltas, sigma, thresh=0.0, thresh_abs=False, skip=0,
          poff=0, qoff=0, interp=False, kr=None, bias=False, hires=False):
     """Whole flct run on C-space arrays; returns (vx, vy, vm), 1 x 1 for
-    sigma = 0."""
+    sigma <= 0."""
     f1, f2 = _f64(f1), _f64(f2)
     nx, ny = f1.shape
-    shape = (1, 1) if sigma == 0 else (nx, ny)
+    shape = (nx, ny) if sigma > 0 else (1, 1)
     vx, vy, vm = (np.empty(shape) for _ in range(3))
     rc = lib().oracle_flct(_ptr(f1), _ptr(f2), nx, ny, deltat, deltas, sigma, thresh,
                            int(thresh_abs), skip, poff, qoff, int(interp),
@@ -185,6 +186,19 @@ def shift_frac2d(arr, delx, dely, transp=1):
     lib().oracle_shift_frac2d(_ptr(arr), float(delx), float(dely), _ptr(out),
                               arr.shape[0], arr.shape[1], transp)
     return out
+
+
+def bias_pass(f1, f2, sigma, i, j, d, kr=None, hires=False):
+    """The -b second pass alone at pixel (i, j) with the offset d = (rows,
+    columns) in surface units; returns the surface shifts (rows, columns)."""
+    f1, f2 = _f64(f1), _f64(f2)
+    sx, sy = ctypes.c_double(), ctypes.c_double()
+    rc = lib().oracle_bias_pass(_ptr(f1), _ptr(f2), f1.shape[0], f1.shape[1], float(sigma),
+                                int(kr is not None), float(kr or 0.0), int(hires), int(i), int(j),
+                                float(d[0]), float(d[1]), ctypes.byref(sx), ctypes.byref(sy))
+    if rc:
+        raise RuntimeError(f"oracle_bias_pass status {rc}")
+    return sx.value, sy.value
 
 
 def resolve_thresh(f1, f2, thresh, thresh_abs=False):

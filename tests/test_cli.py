@@ -123,6 +123,12 @@ def test_flct_cli_sigma0_writes_one_velocity(tmp_path, ctx):
     for a, b in zip(r, g):
         assert a.shape == (1, 1) and np.array_equal(a, f32(b))
     assert abs(r[0][0, 0] + 2) < 1e-5 and abs(r[1][0, 0] - 1) < 1e-5
+    # a negative sigma takes the same branch (flct.c:357-365)
+    out2 = tmp_path / "out2.dat"
+    rc, text = run(FLCT, str(inp), str(out2), "1", "1", "-5", "-k", "0.5")
+    assert rc == 0, text
+    for a, b in zip(vcimage.read3(out2), r):
+        assert a.shape == (1, 1) and np.array_equal(a, b)
 
 
 @pytest.mark.gpu

@@ -294,6 +294,11 @@ def test_sigma0_ignores_threshold_skip_and_takes_device_arrays(ctx):
         g = ctx.track(f1, f2, 1.0, 1.0, 0.0, **kw)
         for a, b in zip(ref, g):
             assert np.array_equal(a, b), kw
+    # a negative sigma is the same whole-image case (flct.c:357-365)
+    for neg in (-3.0, -0.25):
+        g = ctx.track(f1, f2, 1.0, 1.0, neg)
+        for a, b in zip(ref, g):
+            assert a.shape == (1, 1) and np.array_equal(a, b), neg
     # device entry point: the velocity goes to element 0 of each output
     d = [ctx.malloc(f1.nbytes) for _ in range(2)] + [ctx.malloc(8) for _ in range(3)]
     try:

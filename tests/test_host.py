@@ -55,10 +55,10 @@ def test_option_checks_match_reference_exit_sites():
     assert _check(128, 128, deltat=1, deltas=1, sigma=8, skip=4, qoff=-4)[0] == L.FLCT_HIP_E_PQ_RANGE
     assert _check(20, 128, deltat=1, deltas=1, sigma=8, skip=20)[0] == L.FLCT_HIP_E_SKIP_TOO_BIG
     assert _check(5, 5, deltat=1, deltas=1, sigma=8)[0] == L.FLCT_HIP_E_BOX
-    # sigma = 0 correlates the whole image (no box); sigma < 0 is an illegal box
+    # sigma <= 0 correlates the whole image (no box, flct.c:357-365, 567-586)
     assert _check(64, 64, deltat=1, deltas=1, sigma=0) == (0, 0)
     assert _check(7, 9, deltat=1, deltas=1, sigma=0, skip=3)[0] == 0
-    assert _check(64, 64, deltat=1, deltas=1, sigma=-3)[0] == L.FLCT_HIP_E_BOX
+    assert _check(64, 64, deltat=1, deltas=1, sigma=-3) == (0, 0)
     assert _check(64, 64, deltat=1, deltas=1, sigma=0, skip=64)[0] == L.FLCT_HIP_E_SKIP_TOO_BIG
     # box sizes beyond 96 are not instantiated
     assert _check(400, 400, deltat=1, deltas=1, sigma=30)[0] == L.FLCT_HIP_E_UNSUPPORTED

@@ -253,6 +253,122 @@ def test_bias_correction_reduces_bias():
     assert np.max(np.abs(vx)) < 1e-10 and np.max(np.abs(vy)) < 1e-10
 
 
+def _direct_estimate(g1, g2):
+    """One cross_cor estimate by direct sums, no FFT: c(m) = sum_x g1(x)
+    g2(x + m) over the box taken periodically (the c2r of conj(A) B), |c|
+    moved by N/2 in both axes, the first maximum in row-major order, the
+    3x3 Taylor fit for interior peaks (paper eqs. 5-8) and the centre for
+    an all-zero surface.  Returns (rows, columns) shifts."""
+    n = g1.shape[0]
+    c = np.empty((n, n))
+    for mx in range(n):
+        for my in range(n):
+            c[mx, my] = np.sum(g1 * np.roll(g2, (-mx, -my), axis=(0, 1)))
+    s = np.roll(np.abs(c), (n // 2, n // 2), axis=(0, 1))
+    px, py = divmod(int(np.argmax(s)), n)
+    if s[px, py] == 0:
+        return 0.0, 0.0
+    dx = dy = 0.0
+    if 0 < px < n - 1 and 0 < py < n - 1:
+        a = s[px - 1:px + 2, py - 1:py + 2]
+        fx, fy = 0.5 * (a[2, 1] - a[0, 1]), 0.5 * (a[1, 2] - a[1, 0])
+        fxx, fyy = a[2, 1] + a[0, 1] - 2 * a[1, 1], a[1, 2] + a[1, 0] - 2 * a[1, 1]
+        fxy = 0.25 * (a[2, 2] + a[0, 0] - a[2, 0] - a[0, 2])
+        den = fxy * fxy - fxx * fyy
+        dx, dy = (fyy * fx - fy * fxy) / den, (fxx * fy - fx * fxy) / den
+    return px + dx - n // 2, py + dy - n // 2
+
+
+def _direct_bias_pixel(f1, f2, sigma, i, j, n):
+    """-b at pixel (i, j) from its definition: the plain estimate d with
+    the Gaussian centred on the pixel, then image 1 windowed about
+    pixel - d/2 and image 2 about pixel + d/2 over the same clamped box with
+    the same box means, and the estimate again.  Returns (vx, vy)."""
+    nx, ny = f1.shape
+    ns = n // 2
+    i0 = nx - n if i + ns >= nx - 1 else max(0, i - ns + 1)
+    j0 = ny - n if j + ns >= ny - 1 else max(0, j - ns + 1)
+    b1, b2 = f1[i0:i0 + n, j0:j0 + n], f2[i0:i0 + n, j0:j0 + n]
+    b1, b2 = b1 - b1.mean(), b2 - b2.mean()
+    r = np.arange(i0, i0 + n, dtype=float)[:, None]
+    q = np.arange(j0, j0 + n, dtype=float)[None, :]
+
+    def win(ci, cj):
+        return np.exp(-((r - ci) / sigma) ** 2 - ((q - cj) / sigma) ** 2)
+
+    d = _direct_estimate(win(i, j) * b1, win(i, j) * b2)
+    e = _direct_estimate(win(i - d[0] / 2, j - d[1] / 2) * b1, win(i + d[0] / 2, j + d[1] / 2) * b2)
+    return e[1], e[0]  # transp = 1 (flct.c:901-905)
+
+
+def test_bias_matches_direct_sum_definition():
+    """-b on boxes 8 and 12 (sigma 2, 3), every pixel of the frame (clamped
+    edge boxes included): the oracle against its definition evaluated by
+    direct O(N^4) correlation sums, at 1e-10 px."""
+    rng = np.random.default_rng(31)
+    for sigma, (nx, ny) in ((2.0, (16, 18)), (3.0, (20, 22))):
+        n = 2 * O.box_nsize(sigma, nx, ny)
+        assert n == 4 * sigma
+        a, b = smooth_pair(rng, nx, ny, shift=(0.8, -1.3), width=1.5)
+        pix = np.arange(nx * ny)
+        vx, vy, vm = O.flct_pixels(a, b, 1.0, 1.0, sigma, pix, bias=True)
+        px, py, _ = O.flct_pixels(a, b, 1.0, 1.0, sigma, pix)
+        moved = 0
+        for p in pix:
+            i, j = divmod(int(p), ny)
+            ex, ey = _direct_bias_pixel(a, b, sigma, i, j, n)
+            assert abs(vx[p] - ex) < 1e-10 and abs(vy[p] - ey) < 1e-10, (sigma, i, j)
+            moved += abs(vx[p] - px[p]) + abs(vy[p] - py[p]) > 1e-3
+        assert moved > 0.5 * pix.size  # the second pass changes most estimates
+
+
+def test_bias_zero_offset_is_the_plain_pass():
+    """The second pass with d = 0 is the plain pass bit for bit."""
+    rng = np.random.default_rng(32)
+    a, b = smooth_pair(rng, 40, 44, shift=(0.6, 0.9))
+    pix = np.array([0, 43, 5 * 44 + 7, 20 * 44 + 21, 39 * 44 + 43, 33 * 44 + 2])
+    vx, vy, _ = O.flct_pixels(a, b, 1.0, 1.0, 4.0, pix)
+    for k, p in enumerate(pix):
+        sx, sy = O.bias_pass(a, b, 4.0, *divmod(int(p), 44), (0.0, 0.0))
+        assert sy == vx[k] and sx == vy[k]
+
+
+def test_bias_exact_on_a_moved_integer_blob():
+    """A zero-sum integer pattern moved by an integer s, wholly inside the
+    box: with d = s the two windows follow the pattern, image 2's windowed
+    box is image 1's moved by s, and the pass returns s (to FFT rounding);
+    the plain pass does not."""
+    nx = ny = 48
+    pat = np.array([[1, -2, 1, 0], [-2, 5, -3, 1], [0, -2, 1, 0]], float)
+    assert pat.sum() == 0
+    i, j = 24, 23
+    for s in ((2, -1), (-3, 2), (1, 1)):
+        f1, f2 = np.zeros((nx, ny)), np.zeros((nx, ny))
+        f1[i - 1:i + 2, j - 1:j + 3] = pat
+        f2[i - 1 + s[0]:i + 2 + s[0], j - 1 + s[1]:j + 3 + s[1]] = pat
+        sx, sy = O.bias_pass(f1, f2, 4.0, i, j, s)
+        assert abs(sx - s[0]) < 1e-10 and abs(sy - s[1]) < 1e-10, s
+        px, py = O.bias_pass(f1, f2, 4.0, i, j, (0.0, 0.0))
+        assert abs(px - s[0]) + abs(py - s[1]) > 1e-3, s
+
+
+def test_bias_swap_negates_velocity():
+    """Swapping the two images under -b negates every velocity whose peaks
+    are interior (the windows of the swapped pair are the original ones
+    exchanged), on a non-uniform flow."""
+    from pyflct_amd import synth
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(33)
+    nx, ny = 64, 64
+    a = gaussian_filter(rng.standard_normal((nx, ny)), 2.0, mode="wrap")
+    u = 0.9 + 0.5 * np.sin(2 * np.pi * np.arange(nx) / nx)
+    b = synth.shift_rows(a, u)
+    v1 = O.flct(a, b, 1.0, 1.0, 5.0, bias=True)
+    v2 = O.flct(b, a, 1.0, 1.0, 5.0, bias=True)
+    assert np.max(np.abs(v1[0] + v2[0])) < 1e-9 and np.max(np.abs(v1[1] + v2[1])) < 1e-9
+    assert np.median(np.abs(v1[0])) > 0.3
+
+
 def test_deltat_deltas_scaling():
     rng = np.random.default_rng(6)
     a, b = smooth_pair(rng, 40, 40)
@@ -330,7 +446,9 @@ def test_sigma0_ignores_threshold_and_skip():
 
 def test_sigma0_subpixel_shift_and_negative_sigma():
     """A band-limited sub-pixel shift is recovered to the accuracy of the
-    quadratic peak fit; a negative sigma is an illegal box (flct.c:573-579)."""
+    quadratic peak fit; a negative sigma takes the same whole-image branch as
+    sigma = 0 (flct.c:357-365 sets sigmaeq0 for any sigma not > 0, and the
+    box rules of flct.c:567-580 run only without it)."""
     from scipy.ndimage import gaussian_filter
     rng = np.random.default_rng(12)
     g = gaussian_filter(rng.standard_normal((96, 80)), 3.0, mode="wrap")
@@ -340,8 +458,10 @@ def test_sigma0_subpixel_shift_and_negative_sigma():
     h2 = _fourier_shift(g, -0.3, 0.45)
     vx2, vy2, _ = O.flct(g, h2, 1.0, 1.0, 0.0)
     assert abs(vx2[0, 0] + vx[0, 0]) < 1e-6 and abs(vy2[0, 0] + vy[0, 0]) < 1e-6
-    with pytest.raises(RuntimeError):
-        O.flct(g, h, 1.0, 1.0, -4.0)
+    for neg in (-4.0, -0.5):
+        got = O.flct(g, h, 1.0, 1.0, neg)
+        for a, b in zip((vx, vy, _), got):
+            assert a.shape == (1, 1) and np.array_equal(a, b)
 
 
 def test_warp_fixture_was_made_in_double_precision(golden):

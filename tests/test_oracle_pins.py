@@ -21,6 +21,17 @@ PLANTS = {
                       T.test_uniform_shift_recovers_known_velocity],
     "PLANT_NYQUIST_NEG": [T.test_make_freq_closed_form],
     "PLANT_HIRES_SCALE": [T.test_hires_peak_on_sampled_quadratic],
+    # -b: offset d instead of d/2, windows moved the wrong way, means
+    # recomputed over the moved windows, only image 2's window moved
+    "PLANT_BIAS_FULL_D": [T.test_bias_matches_direct_sum_definition,
+                          T.test_bias_exact_on_a_moved_integer_blob],
+    "PLANT_BIAS_SIGN": [T.test_bias_matches_direct_sum_definition,
+                        T.test_bias_exact_on_a_moved_integer_blob],
+    "PLANT_BIAS_REMEAN": [T.test_bias_matches_direct_sum_definition,
+                          T.test_bias_zero_offset_is_the_plain_pass],
+    "PLANT_BIAS_ONE_IMAGE": [T.test_bias_matches_direct_sum_definition,
+                             T.test_bias_exact_on_a_moved_integer_blob,
+                             T.test_bias_swap_negates_velocity],
 }
 
 
