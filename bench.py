@@ -10,6 +10,7 @@ import collections
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -83,6 +84,37 @@ def form_roofline(form, r, nbox):
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """Model name and thread count of this host's CPUs, and whether a host
+    FFTW3 library is installed (ldconfig -p; ROCm's GPU libhipfftw does not
+    count)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    fftw = None
+    try:
+        out = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            name = line.strip().split(" ", 1)[0]
+            if name.startswith("libfftw3.so"):
+                fftw = line.split("=>")[-1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "threads_available": avail,
+            "host_libfftw3": fftw}
 
 
 def cpu_baseline(f1, f2, sigma, budget_s=20.0):
@@ -280,8 +312,12 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle baseline")
         v, sample = cpu_baseline(f1, f2, args.sigma, args.cpu_budget)
+        hc = host_cpu()
         cpu = {"value": round(v, 1), "unit": "tracked px/s", "cores": 1, "kind": "port",
-               "sample": sample}
+               "sample": sample, "cpu_model": hc["cpu_model"], "nproc": hc["nproc"],
+               "threads_available": hc["threads_available"],
+               "fft": ("the port's own mixed-radix FFT; host libfftw3: "
+                       + (hc["host_libfftw3"] or "none installed (ldconfig -p)"))}
 
     out = {
         "metric": METRIC,

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -56,8 +57,9 @@ struct flct_hip_ctx {
   int seg_n = 0;          // box size of the split tables in segtab
   double seg_sigma = 0.;  // window width of the split tables
   int bf16_kernel = 0;    // FLCT_HIP_BF16_KERNEL: 0 automatic, 1 per-pixel kernels only
-  // kernel path by (precision, box size), chosen by flct_hip_autotune
-  std::map<std::pair<int, int>, int> tuned;
+  // kernel path by (precision, box size, bias), chosen by flct_hip_autotune;
+  // -h always runs the FP64 kernels and is never stored
+  std::map<std::tuple<int, int, int>, int> tuned;
   int tune_gen = 0;       // bumped by every tuning (part of the batch graph key)
   int force_path = -1;    // path under test while tuning
   int last_path = 0;
@@ -371,7 +373,7 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Pla
   if (force_path >= 0) {
     pl->path = force_path;
   } else {
-    const auto it = c->tuned.find({o->precision, N});
+    const auto it = c->tuned.find(std::make_tuple(o->precision, N, o->bias ? 1 : 0));
     pl->path = (it != c->tuned.end()) ? it->second
                                       : default_path(o->precision, N, nx, ny, o->bias, c->bf16_kernel);
   }
@@ -842,9 +844,12 @@ extern "C" const char* flct_hip_last_path(flct_hip_ctx* c) {
 extern "C" int flct_hip_set_path(flct_hip_ctx* c, int precision, int nbox, const char* path) {
   if (!c || nbox < 4 || nbox % 4 != 0 || nbox / 4 > kTrackMaxM) return FLCT_HIP_E_ARG;
   if (precision < FLCT_HIP_FP64 || precision > FLCT_HIP_SPLIT_BF16) return FLCT_HIP_E_ARG;
-  const std::pair<int, int> key{precision, nbox};
+  // one choice for calls with and without -b (a -b call the row-segment
+  // kernel cannot serve runs the per-pixel one)
+  const auto key = std::make_tuple(precision, nbox, 0), keyb = std::make_tuple(precision, nbox, 1);
   if (!path || !*path) {  // back to the untuned default
     c->tuned.erase(key);
+    c->tuned.erase(keyb);
     ++c->tune_gen;
     return FLCT_HIP_OK;
   }
@@ -858,6 +863,7 @@ extern "C" int flct_hip_set_path(flct_hip_ctx* c, int precision, int nbox, const
   for (int q : path_candidates(precision, nbox, 1 << 20, 1 << 20, 0, 0)) ok = ok || (q == p);
   if (!ok) return FLCT_HIP_E_ARG;
   c->tuned[key] = p;
+  c->tuned[keyb] = p;
   ++c->tune_gen;
   return FLCT_HIP_OK;
 }
@@ -895,8 +901,11 @@ extern "C" int flct_hip_autotune(flct_hip_ctx* c, const double* f1, const double
       best_ms = t;
     }
   }
-  c->tuned[{o->precision, pl.N}] = best;
-  c->tune_gen++;
+  // -h forces the FP64 kernels whatever is stored: nothing to keep
+  if (!o->hires) {
+    c->tuned[std::make_tuple(o->precision, pl.N, o->bias ? 1 : 0)] = best;
+    c->tune_gen++;
+  }
   c->last_path = best;
   return FLCT_HIP_OK;
 }

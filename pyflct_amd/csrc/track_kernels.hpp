@@ -67,7 +67,7 @@ constexpr double kFitTol32 = 1e-2;
 
 // Slots (pixels tracked together) per block, from a sweep of 1, 2, 3, 4, 6
 // and 8 slots over every box size at 2048^2 (every pixel tracked,
-// pyflct_amd/tune/sweep_slots.py), where the best count beat the default
+// one library per count built with TUNE_FLAGS=-DFLCT_TRACK_SLOTS=n), where the best count beat the default
 // rule below by 4 % or more; 0: default rule.
 constexpr int tuned_slots(int M, bool fp32) {
   if (fp32) {

@@ -35,12 +35,14 @@ def make_freq(n):
 
 
 def add_blobs(img, rng, count, amp_fn, width_lo, width_hi, bipolar=False, mask=None):
+    """Adds `count` blobs; with `mask`, a centre the mask rejects is drawn
+    again, so all `count` blobs land where the mask accepts."""
     nx, ny = img.shape
     for _ in range(count):
         s = rng.uniform(width_lo, width_hi)
         ci, cj = rng.uniform(0, nx), rng.uniform(0, ny)
-        if mask is not None and not mask(ci, cj):
-            continue
+        while mask is not None and not mask(ci, cj):
+            ci, cj = rng.uniform(0, nx), rng.uniform(0, ny)
         a = amp_fn(rng)
         parts = [(ci, cj, a)]
         if bipolar:

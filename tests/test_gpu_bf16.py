@@ -226,6 +226,15 @@ def test_paths_and_autotune():
                 assert np.array_equal(got[2], r[2])
                 rms, _ = _errs(r, got)
                 assert rms <= (RMS_TOL if form == "split_bf16" else 2e-3), (form, best, rms)
+            # tuning with -h (FP64 kernels only) or -b stores nothing for the
+            # plain calls of that form and box size
+            c.set_path("split_bf16", 40, None)
+            c.autotune(d[0], d[1], 384, 384, c.make_opts(1.0, 1.0, 10.0, precision="split_bf16",
+                                                        hires=True))
+            c.autotune(d[0], d[1], 384, 384, c.make_opts(1.0, 1.0, 10.0, precision="split_bf16",
+                                                        bias=True))
+            c.track(f1, f2, 1.0, 1.0, 10.0, precision="split_bf16")
+            assert c.last_path() == "bf16-segment"
         finally:
             for p_ in d:
                 c.free(p_)

@@ -96,32 +96,3 @@ def test_vcimage_round_trip_and_header(tmp_path, golden):
     bad.write_bytes(b"\0" * 16)
     with pytest.raises(vcimage.VcimageError):
         vcimage.read(bad)
-
-
-def test_slot_sweep_table(tmp_path, capsys):
-    """The slot sweep's table step keeps a count only where it beats the
-    default build by the margin."""
-    from pyflct_amd.tune import sweep_slots
-    f = tmp_path / "sweep.txt"
-    f.write_text("default fp32 28 35.5\nlibflct_hip_p1.so fp32 28 24.9\nlibflct_hip_p2.so fp32 28 26.9\n"
-                 "default fp64 64 212.1\nlibflct_hip_p4.so fp64 64 211.2\n# comment\n")
-    sweep_slots.table(str(f), 0.04)
-    out = capsys.readouterr().out.strip().splitlines()
-    assert out == ["fp32 box 28 (M = 7): 1 slots, 35.5 -> 24.9 ms"]
-
-
-def test_lds_bank_table(tmp_path):
-    """The bank probe's table: cycles per load relative to the first
-    (conflict-free) pattern, which also carries the LDS fill of each launch."""
-    from pyflct_amd.tune import lds_bank_table as T
-    loads = T.LOADS
-    hdr = "Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n"
-    rows = []
-    for did, act, bc in ((3, 1000 + loads, 0), (5, 1000 + 7 * loads, 4 * loads), (4, 9, 9)):
-        kname = "k_probe(int const*, int, unsigned int*)" if did != 4 else "other"
-        rows.append(f'{did},"{kname}",SQ_LDS_IDX_ACTIVE,{act}')
-        rows.append(f'{did},"{kname}",SQ_LDS_BANK_CONFLICT,{bc}')
-    (tmp_path / "c.csv").write_text(hdr + "\n".join(rows) + "\n")
-    (tmp_path / "p.log").write_text("noise\npattern: b32 consecutive\npattern: read2 pairs\n")
-    got = T.table(str(tmp_path / "c.csv"), str(tmp_path / "p.log"))
-    assert got == [("b32 consecutive", 1.0, 0.0), ("read2 pairs", 7.0, 4.0)]
