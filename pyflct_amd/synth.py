@@ -104,17 +104,20 @@ def rotation_field(n, degrees):
 def fourier_mode_field(n, rng, nmodes=8, lam=(64.0, 512.0), vmax=1.5):
     """Smooth random velocity: a sum of random Fourier modes with
     wavelengths in lam, scaled to max |v| = vmax."""
-    y, x = np.mgrid[0:n, 0:n].astype(np.float64)
+    t = np.arange(n, dtype=np.float64)  # x = column j, y = row i
     vx = np.zeros((n, n))
     vy = np.zeros((n, n))
     for _ in range(nmodes):
         L = rng.uniform(*lam)
         th = rng.uniform(0, 2 * np.pi)
         kx, ky = np.cos(th) / L, np.sin(th) / L
-        ph = 2 * np.pi * (kx * x + ky * y) + rng.uniform(0, 2 * np.pi)
+        a = 2 * np.pi * kx * t                                 # phase along x
+        b = 2 * np.pi * ky * t + rng.uniform(0, 2 * np.pi)    # phase along y
+        # sin(a + b) as two outer products (row index y, column index x)
+        mode = np.outer(np.cos(b), np.sin(a)) + np.outer(np.sin(b), np.cos(a))
         ax, ay = rng.standard_normal(2)
-        vx += ax * np.sin(ph)
-        vy += ay * np.sin(ph)
+        vx += ax * mode
+        vy += ay * mode
     m = np.max(np.hypot(vx, vy))
     return vx * (vmax / m), vy * (vmax / m)
 

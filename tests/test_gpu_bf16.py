@@ -299,3 +299,82 @@ def test_pixel_kernels_every_box_size(monkeypatch):
             assert rms <= RMS_TOL and mx <= MAX_TOL, (nbox, rms, mx)
     finally:
         c.close()
+
+
+# ------------------------------------------------ against the CPU oracle
+def _oracle_sample(nx, ny, rng, nrand=300):
+    """Frame corners, every 5th frame-edge pixel and random pixels."""
+    i, j = np.mgrid[0:nx, 0:ny]
+    edge = np.flatnonzero((i == 0) | (i == nx - 1) | (j == 0) | (j == ny - 1))
+    return np.unique(np.concatenate([edge[::5], [0, ny - 1, (nx - 1) * ny, nx * ny - 1],
+                                     rng.choice(nx * ny, nrand, replace=False)]))
+
+
+def _vs_oracle(g, o, pix, where):
+    vx, vy, vm = (a.ravel()[pix] for a in g)
+    assert np.array_equal(vm, o[2]), where
+    e = np.hypot(vx - o[0], vy - o[1])
+    rms = float(np.sqrt(np.mean(e ** 2)))
+    assert rms <= RMS_TOL and e.max() <= MAX_TOL, (where, rms, float(e.max()))
+    return rms
+
+
+# one box per kernel family: 20 (single-tile row segment; dense kernel
+# NP = 32), 32 (single-tile row segment at its limit), 48 (row segment;
+# dense kernel NP = 64) and 64 (row segment; dense kernel NP = 64, radix-2
+# edge kernel); every BF16 path the box allows, and the default path
+@pytest.mark.parametrize("sigma", [5.0, 8.0, 12.0, 15.0])
+def test_bf16_paths_against_oracle(ctx, sigma):
+    import oracle_lib as O
+    f1, f2, _ = synth.cfg4(n=320)
+    nx, ny = f1.shape
+    nbox = ctx.box_size(nx, ny, sigma)
+    pix = _oracle_sample(nx, ny, np.random.default_rng(int(sigma)))
+    o = O.flct_pixels(f1, f2, 1.0, 1.0, sigma, pix)
+    for path in (None, "bf16-segment", "bf16-pixel"):
+        ctx.set_path("split_bf16", nbox, path)
+        try:
+            g = ctx.track(f1, f2, 1.0, 1.0, sigma, precision="split_bf16")
+            if path is not None:
+                assert ctx.last_path() == path
+            _vs_oracle(g, o, pix, f"box {nbox} {ctx.last_path()}")
+        finally:
+            ctx.set_path("split_bf16", nbox, None)
+
+
+def test_bf16_full_disk_4096_against_oracle(ctx):
+    """The 4096^2, sigma = 15 full-disk pair in split-BF16: sampled pixels
+    (interior, limb, frame corners and edges) against the oracle."""
+    import oracle_lib as O
+    f1, f2, _ = synth.cfg4()
+    n = f1.shape[0]
+    g = ctx.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
+    assert ctx.last_path() == "bf16-segment"
+    rng = np.random.default_rng(14)
+    pix = np.unique(np.concatenate([_oracle_sample(n, n, rng, 250)[::3],
+                                    np.arange(n * 2048, n * 2048 + n, 61),  # across the disk
+                                    n * np.arange(0, n, 61) + 2048]))
+    o = O.flct_pixels(f1, f2, 1.0, 1.0, 15.0, pix)
+    rms = _vs_oracle(g, o, pix, "4096^2 sigma 15")
+    assert rms <= 5e-5, rms
+
+
+@pytest.mark.parametrize("form", ["fp32", "split_bf16"])
+def test_pedestal_image_against_oracle(ctx, form):
+    """An intensity-like pair with a large constant level (1e4 plus 1 %
+    structure): the box means remove the level in FP64 before any
+    reduced-precision arithmetic, so the error contract holds."""
+    import oracle_lib as O
+    from conftest import smooth_pair
+    rng = np.random.default_rng(15)
+    a, b = smooth_pair(rng, 240, 260, shift=(0.7, -0.4), width=2.5)
+    s = 100.0 / np.std(a)
+    a, b = 1e4 + s * a, 1e4 + s * b
+    for sigma in (6.0, 12.0, 15.0):
+        ref = ctx.track(a, b, 1.0, 1.0, sigma)
+        g = ctx.track(a, b, 1.0, 1.0, sigma, precision=form)
+        assert np.array_equal(g[2], ref[2])
+        rms, mx = _errs(ref, g)
+        assert rms <= RMS_TOL and mx <= MAX_TOL, (form, sigma, rms, mx)
+        pix = _oracle_sample(*a.shape, rng, 150)
+        _vs_oracle(g, O.flct_pixels(a, b, 1.0, 1.0, sigma, pix), pix, f"{form} sigma {sigma}")

@@ -1,0 +1,139 @@
+"""The uneven multi-pair workload at its stated size (GPU): 64 image pairs,
+sizes uniform in {256, 512, 1024, 2048}^2, sigma uniform in 4..20 (boxes
+16..84), each a magnetogram warped by its own smooth random velocity field
+(8 modes, wavelengths 64-512 px, max |v| 1.5 px), the generator and seed of
+bench.py's uneven-batch workload.
+
+All 64 pairs go through one flct_hip_track_batch call per precision form.
+FP64 is checked against the CPU oracle on 200 sampled pixels per pair
+(frame corners and edges included) at 1e-8 px with vm bit-exact; FP32 and
+split-BF16 against FP64 on every pixel and against the oracle on the same
+samples (RMS <= 1e-4 px, vm bit-exact); image 1 warped by the recovered
+velocities is compared with image 2.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from pyflct_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ("fp64", "fp32", "split_bf16")
+RMS_TOL = 1e-4   # px, precision forms
+MAX_TOL = 5e-3   # px, any pixel of a precision form
+NSAMPLE = 200
+
+
+def _sample(n, rng):
+    """Corners, 36 frame-edge pixels and random interior ones."""
+    edge = np.concatenate([np.arange(n), (n - 1) * n + np.arange(n),
+                           np.arange(n) * n, np.arange(n) * n + n - 1])
+    corners = np.array([0, n - 1, (n - 1) * n, n * n - 1])
+    pick = rng.choice(edge, 36, replace=False)
+    rest = rng.choice(n * n, NSAMPLE - 40, replace=False)
+    return np.unique(np.concatenate([corners, pick, rest]))
+
+
+@pytest.fixture(scope="module")
+def cfg5(ctx):
+    rng = np.random.default_rng(20261207)  # the pairs of bench.py's uneven batch
+    srng = np.random.default_rng(5)        # the sampled pixels
+    pairs, bufs = [], []
+    try:
+        for k in range(64):
+            n = int(rng.choice([256, 512, 1024, 2048]))
+            sigma = float(rng.uniform(4.0, 20.0))
+            f1, dx, dy = synth.uneven_inputs(n, seed=int(rng.integers(1 << 30)))
+            d = {key: ctx.malloc(f1.nbytes) for key in ("f1", "f2", "dx", "dy")}
+            bufs.append(d)
+            ctx.upload(d["f1"], f1)
+            ctx.upload(d["dx"], dx)
+            ctx.upload(d["dy"], dy)
+            ctx.warp_device(d["f1"], d["dx"], d["dy"], d["f2"], n, n)
+            for form in FORMS:
+                for o in ("vx", "vy", "vm"):
+                    d[f"{form}_{o}"] = ctx.malloc(f1.nbytes)
+            pairs.append({"n": n, "sigma": sigma, "d": d, "f1": f1,
+                          "f2": ctx.download(np.empty((n, n)), d["f2"]),
+                          "pix": _sample(n, srng)})
+        stats = {}
+        for form in FORMS:
+            desc = [(p["d"]["f1"], p["d"]["f2"], p["d"][f"{form}_vx"], p["d"][f"{form}_vy"],
+                     p["d"][f"{form}_vm"], p["n"], p["n"],
+                     ctx.make_opts(1.0, 1.0, p["sigma"], precision=form)) for p in pairs]
+            stats[form] = ctx.track_batch_device(desc)
+        for k, p in enumerate(pairs):
+            n, d = p["n"], p["d"]
+            out = {form: tuple(ctx.download(np.empty((n, n)), d[f"{form}_{o}"])
+                               for o in ("vx", "vy", "vm")) for form in FORMS}
+            ref = out["fp64"]
+            p["sampled"] = {form: tuple(v.ravel()[p["pix"]] for v in out[form]) for form in FORMS}
+            p["tracked"] = {form: stats[form][k]["tracked"] for form in FORMS}
+            p["vs_fp64"] = {}
+            for form in FORMS[1:]:
+                g = out[form]
+                e = np.hypot(g[0] - ref[0], g[1] - ref[1])
+                p["vs_fp64"][form] = (bool(np.array_equal(g[2], ref[2])),
+                                      float(np.sqrt(np.mean(e ** 2))), float(e.max()),
+                                      bool(np.isfinite(g[0]).all() and np.isfinite(g[1]).all()))
+            # image 1 warped by the split-BF16 velocities, into the dx buffer
+            ctx.warp_device(d["f1"], d["split_bf16_vx"], d["split_bf16_vy"], d["dx"], n, n)
+            w = ctx.download(np.empty((n, n)), d["dx"])
+            m = slice(n // 8, n - n // 8)
+            b = p["f2"][m, m]
+            scale = np.sqrt(np.mean(b ** 2))
+            p["residual"] = float(np.sqrt(np.mean((w[m, m] - b) ** 2)) / scale)
+            p["unwarped"] = float(np.sqrt(np.mean((p["f1"][m, m] - b) ** 2)) / scale)
+        yield pairs
+    finally:
+        for d in bufs:
+            for ptr_ in d.values():
+                ctx.free(ptr_)
+
+
+def test_cfg5_layout(cfg5):
+    """Every size and most box sizes of the workload occur; every pixel is
+    tracked in every form (no threshold, no skip)."""
+    assert len(cfg5) == 64
+    assert {p["n"] for p in cfg5} == {256, 512, 1024, 2048}
+    boxes = {O.box_nsize(p["sigma"], p["n"], p["n"]) * 2 for p in cfg5}
+    assert min(boxes) == 16 and max(boxes) >= 80 and len(boxes) >= 14
+    for p in cfg5:
+        for form in FORMS:
+            assert p["tracked"][form] == p["n"] * p["n"], (p["n"], p["sigma"], form)
+
+
+def test_cfg5_fp64_matches_oracle(cfg5):
+    for p in cfg5:
+        o = O.flct_pixels(p["f1"], p["f2"], 1.0, 1.0, p["sigma"], p["pix"])
+        vx, vy, vm = p["sampled"]["fp64"]
+        where = f"{p['n']}^2 sigma {p['sigma']:.2f}"
+        assert np.array_equal(vm, o[2]), where
+        assert np.max(np.abs(vx - o[0])) <= 1e-8 and np.max(np.abs(vy - o[1])) <= 1e-8, where
+
+
+def test_cfg5_precision_forms_against_fp64_and_oracle(cfg5):
+    worst = {form: 0.0 for form in FORMS[1:]}
+    for p in cfg5:
+        o = O.flct_pixels(p["f1"], p["f2"], 1.0, 1.0, p["sigma"], p["pix"])
+        where = f"{p['n']}^2 sigma {p['sigma']:.2f}"
+        for form in FORMS[1:]:
+            same_vm, rms, mx, finite = p["vs_fp64"][form]
+            assert same_vm and finite, (form, where)
+            assert rms <= RMS_TOL and mx <= MAX_TOL, (form, where, rms, mx)
+            vx, vy, vm = p["sampled"][form]
+            assert np.array_equal(vm, o[2]), (form, where)
+            e = np.hypot(vx - o[0], vy - o[1])
+            assert np.sqrt(np.mean(e ** 2)) <= RMS_TOL and e.max() <= MAX_TOL, (form, where)
+            worst[form] = max(worst[form], rms)
+    print(f"cfg5 worst per-pair RMS against FP64: {worst}")
+
+
+def test_cfg5_warp_residual(cfg5):
+    """Image 1 warped by the recovered split-BF16 velocities is closer to
+    image 2 than image 1 itself (interior, relative RMS)."""
+    for p in cfg5:
+        assert p["residual"] < 0.5 * p["unwarped"], (p["n"], p["sigma"], p["residual"],
+                                                     p["unwarped"])
+    assert np.median([p["residual"] / p["unwarped"] for p in cfg5]) < 0.4

@@ -248,7 +248,7 @@ def test_error_codes(ctx):
     cases = [({"sigma": 4.0, "skip": 50}, L.FLCT_HIP_E_SKIP_TOO_BIG),
              ({"sigma": 4.0, "skip": 4, "poff": 5}, L.FLCT_HIP_E_PQ_RANGE),
              ({"sigma": 4.0, "skip": -1}, L.FLCT_HIP_E_SKIP_NONPOS),
-             ({"sigma": -3.0}, L.FLCT_HIP_E_BOX),
+             ({"sigma": 0.5}, L.FLCT_HIP_E_BOX),  # (int)sigma = 0: box 0 (flct.c:570-579)
              ({"sigma": 0.0, "skip": 40}, L.FLCT_HIP_E_SKIP_TOO_BIG)]
     for kw, code in cases:
         sig = kw.pop("sigma")
