@@ -32,18 +32,17 @@ __global__ void k_absmax(const double* __restrict__ f, long long n,
 // reference's row-major loop.
 // Each output is summed in the same order as a plain loop over c (pass 1)
 // and r (pass 2), so the means do not depend on the launch shape.
-// Pass 1 stages kMeanTile + N - 1 values of one image row in LDS and each
-// thread sums its N from there; pass 2 gives each thread kMeanRows
+// Pass 1 stages kMeanTile + N - 1 values of one image row in LDS (dynamic,
+// (kMeanTile + N - 1) doubles) and each thread sums its N from there; pass 2 gives each thread kMeanRows
 // consecutive box origins of one column and adds every loaded row sum to the
 // origins whose box holds it, so each value is loaded once per thread
 // instead of once per output.
 constexpr int kMeanTile = 256;     // pass-1 outputs per block (= threads)
 constexpr int kMeanRows = 8;       // pass-2 origins per thread
-constexpr int kMeanMaxN = 96;      // largest box
 
 __global__ void __launch_bounds__(kMeanTile) k_rowsum(const double* __restrict__ f, int nx, int ny, int N,
                                                       double* __restrict__ hs) {
-  __shared__ double tile[kMeanTile + kMeanMaxN - 1];
+  extern __shared__ double tile[];
   const int ns = ny - N + 1;
   const int nbx = (ns + kMeanTile - 1) / kMeanTile;
   const int r = (int)(blockIdx.x / nbx);

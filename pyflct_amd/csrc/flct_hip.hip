@@ -20,10 +20,10 @@
 #include "nufft_kernels.hpp"
 #include "fft_rows.hpp"
 #include "track_launch.hpp"
+#include "track_big.hpp"
 
 using namespace flct;
 
-static_assert(4 * kTrackMaxM <= kMeanMaxN, "the box-mean row tile must hold the largest box");
 
 namespace {
 
@@ -42,7 +42,7 @@ struct flct_hip_ctx {
   size_t lds_per_block = 0;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   DevBuf f1, f2, vx, vy, vm, list, count, gw, filt, absmax, wtmp, wF, wtw1, wtw2, w3, w4, w5,
-      hs1, hs2, mean1, mean2, flags, fcount, segtab, elist, ecount, nuT, nuR, nuTab;
+      hs1, hs2, mean1, mean2, flags, fcount, segtab, elist, ecount, nuT, nuR, nuTab, bigp;
   // tables of the row transforms (fft_rows.hpp) by (length, sign)
   struct DftTab {
     int n, sign, logm, bluestein;
@@ -50,7 +50,7 @@ struct flct_hip_ctx {
   };
   std::vector<DftTab> dft_tabs;
   // batch: own scratch, table arena and counters, and the cached graph
-  DevBuf bscr[7], barena, bcount;
+  DevBuf bscr[8], barena, bcount;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<unsigned char> gkey;  // pairs the cached graph was captured for
@@ -153,7 +153,7 @@ extern "C" int flct_hip_destroy(flct_hip_ctx* c) {
                     &c->filt, &c->absmax, &c->wtmp, &c->wF, &c->wtw1, &c->wtw2, &c->w3,
                     &c->w4, &c->w5, &c->hs1, &c->hs2, &c->mean1, &c->mean2, &c->flags,
                     &c->fcount, &c->segtab, &c->elist, &c->ecount, &c->nuT, &c->nuR,
-                    &c->nuTab};
+                    &c->nuTab, &c->bigp};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (DevBuf& b : c->bscr)
@@ -243,7 +243,7 @@ extern "C" int flct_hip_check_opts(int nx, int ny, const flct_hip_opts* o, int* 
   const int rc = resolve(o, nx, ny, &r);
   if (rc) return rc;
   if (nbox) *nbox = r.N;
-  if (!r.whole && (r.N % 4 != 0 || r.N / 4 > kTrackMaxM)) return FLCT_HIP_E_UNSUPPORTED;
+  if (!r.whole && (r.N % 4 != 0 || r.N > kBigMaxN)) return FLCT_HIP_E_UNSUPPORTED;
   if (o->precision < FLCT_HIP_FP64 || o->precision > FLCT_HIP_SPLIT_BF16) return FLCT_HIP_E_ARG;
   if (o->deltat == 0.) return FLCT_HIP_E_ARG;
   return FLCT_HIP_OK;
@@ -251,7 +251,15 @@ extern "C" int flct_hip_check_opts(int nx, int ny, const flct_hip_opts* o, int* 
 
 // Kernel paths of the tracker.  A precision form may run on any path whose
 // error is within the form's contract: FP64 serves every form.
-enum Path { PATH_FP64 = 0, PATH_FP32 = 1, PATH_BF16_SEG = 2, PATH_BF16_PIXEL = 3 };
+// PATH_FP64_PLANE: boxes above 96, every form (k_track_big, FP64)
+enum Path {
+  PATH_FP64 = 0,
+  PATH_FP32 = 1,
+  PATH_BF16_SEG = 2,
+  PATH_BF16_PIXEL = 3,
+  PATH_FP64_PLANE = 4
+};
+constexpr int kQuadMaxN = 4 * kTrackMaxM;  // largest box of the quad-FFT kernels
 
 static const char* path_name(int p) {
   switch (p) {
@@ -259,6 +267,7 @@ static const char* path_name(int p) {
     case PATH_FP32: return "fp32-fft";
     case PATH_BF16_SEG: return "bf16-segment";
     case PATH_BF16_PIXEL: return "bf16-pixel";
+    case PATH_FP64_PLANE: return "fp64-plane";
     default: return "none";
   }
 }
@@ -275,6 +284,7 @@ constexpr int kBf16PixelMaxBox = 72;
 // unless default_path says otherwise)
 static std::vector<int> path_candidates(int precision, int N, int nx, int ny, int bias,
                                         int hires) {
+  if (N > kQuadMaxN) return {PATH_FP64_PLANE};  // boxes above 96: FP64 in every form
   if (hires) return {PATH_FP64};  // the hires refinement runs in the FP64 kernels
   if (precision == FLCT_HIP_FP32) return {PATH_FP32, PATH_FP64};
   if (precision == FLCT_HIP_SPLIT_BF16) {
@@ -296,6 +306,7 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
 // FP64 kernels and 2.9-3.7x faster than the dense-DFT BF16 kernel there
 // (2048^2, boxes 68..84).
 static int default_path(int precision, int N, int nx, int ny, int bias, int bf16_kernel) {
+  if (N > kQuadMaxN) return PATH_FP64_PLANE;
   if (precision == FLCT_HIP_FP32) return PATH_FP32;
   if (precision == FLCT_HIP_SPLIT_BF16) {
     if (bf16_kernel) return (N <= kBf16PixelMaxBox) ? PATH_BF16_PIXEL : PATH_FP32;
@@ -319,6 +330,7 @@ struct Plan {
   bool seg = false;       // split-BF16 row-segment kernel for interior columns
   int path = PATH_FP64;   // kernel path
   int ixc = 0, iyc = 0;   // skip-fill lattice size
+  long long big_grid = 0; // blocks of k_track_big (boxes above 96)
   std::vector<double> gw, filt;
 };
 
@@ -337,7 +349,7 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Pla
   pl->o = *o;
   pl->N = pl->r.N;
   pl->M = pl->N / 4;
-  if (pl->N % 4 != 0 || pl->M < 1 || pl->M > kTrackMaxM) return FLCT_HIP_E_UNSUPPORTED;
+  if (pl->N % 4 != 0 || pl->M < 1 || pl->N > kBigMaxN) return FLCT_HIP_E_UNSUPPORTED;
   if ((long long)nx * ny > 0x7fffffffLL) return FLCT_HIP_E_UNSUPPORTED;
   const int N = pl->N;
   // window table gw[d + N - 1] = exp(-(sigminv d)^2) (flct.c:652-663, separable)
@@ -379,6 +391,11 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Pla
   }
   if (pl->path == PATH_BF16_SEG && !seg_supported(N, nx, ny, o->bias)) pl->path = PATH_BF16_PIXEL;
   if (o->hires) pl->path = PATH_FP64;  // -h: the FP64 kernels carry the hires refinement
+  if (N > kQuadMaxN) {
+    // boxes above 96 run k_track_big in every form, -h and -b included
+    pl->path = PATH_FP64_PLANE;
+    pl->big_grid = big_grid(c->num_cu, N, (long long)nx * ny);
+  }
   pl->seg = (pl->path == PATH_BF16_SEG);
   if (pl->r.skipon && o->interp) {
     int ixc = 0, iyc = 0;
@@ -399,6 +416,7 @@ struct Tables {
   const double* gw = nullptr;
   const double* filt = nullptr;
   const void* segtab = nullptr;
+  BigTab big_fwd, big_inv;  // transform tables of boxes above 96
 };
 
 // Device scratch of one call, reused by the pairs of a batch in stream order.
@@ -410,10 +428,13 @@ struct Scratch {
   double* mean2 = nullptr;
   int* flags = nullptr;  // split-BF16 and FP32 pixels for FP64 re-evaluation
   int* elist = nullptr;  // edge-column pixels of the row-segment path
+  void* planes = nullptr;  // k_track_big: per block an N x N complex and an N x N real plane
 };
 
-// Scratch bytes of one plan: list, hs1, hs2, mean1, mean2, flags, elist
-void scratch_sizes(const Plan& pl, size_t sz[7]) {
+constexpr int kNumScratch = 8;
+
+// Scratch bytes of one plan: list, hs1, hs2, mean1, mean2, flags, elist, planes
+void scratch_sizes(const Plan& pl, size_t sz[kNumScratch]) {
   const long long npix = (long long)pl.nx * pl.ny;
   const int N = pl.N, nsx = pl.ny - N + 1, nrx = pl.nx - N + 1;
   sz[0] = pl.sparse ? npix * sizeof(int) : 0;
@@ -421,6 +442,7 @@ void scratch_sizes(const Plan& pl, size_t sz[7]) {
   sz[3] = sz[4] = (size_t)nrx * nsx * 8;
   sz[5] = (pl.path != PATH_FP64) ? npix * sizeof(int) : 0;
   sz[6] = pl.seg ? (pl.sparse ? npix : (long long)pl.nx * (N - 1)) * sizeof(int) : 0;
+  sz[7] = (pl.path == PATH_FP64_PLANE) ? (size_t)pl.big_grid * 24 * (size_t)N * N : 0;
 }
 
 // Enqueues the tracking pipeline of one pair on lc.stream: no host
@@ -436,6 +458,7 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
   const int nx = pl.nx, ny = pl.ny, N = pl.N, M = pl.M;
   const long long npix = (long long)nx * ny;
   const bool bf16 = (pl.path == PATH_BF16_SEG || pl.path == PATH_BF16_PIXEL);
+  const bool big = (pl.path == PATH_FP64_PLANE);
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, counts, 4);
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned int*>(absmax), 4);
   // image maxima: relative threshold (flct.c:594-629) and the split-BF16 scaling
@@ -465,8 +488,9 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
   // box means by box origin (flct.c:815-827)
   const long long nbx = (ny - N + 1 + kMeanTile - 1) / kMeanTile;
   const long long nrb = (nx - N + 1 + kMeanRows - 1) / kMeanRows, nby = (ny - N + 1 + 255) / 256;
-  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)(nx * nbx)), dim3(kMeanTile), 0, s, f1, nx, ny, N, sc.hs1);
-  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)(nx * nbx)), dim3(kMeanTile), 0, s, f2, nx, ny, N, sc.hs2);
+  const size_t rs_lds = (size_t)(kMeanTile + N - 1) * sizeof(double);
+  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)(nx * nbx)), dim3(kMeanTile), rs_lds, s, f1, nx, ny, N, sc.hs1);
+  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)(nx * nbx)), dim3(kMeanTile), rs_lds, s, f2, nx, ny, N, sc.hs2);
   hipLaunchKernelGGL(k_boxmean, dim3((unsigned)(nrb * nby)), dim3(256), 0, s,
                      (const double*)sc.hs1, nx, ny, N, sc.mean1);
   hipLaunchKernelGGL(k_boxmean, dim3((unsigned)(nrb * nby)), dim3(256), 0, s,
@@ -502,7 +526,36 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
   ta.flag_count = refine ? counts + 1 : nullptr;
   if (ev) HIPCHK(hipEventRecord(ev[1], s));
   int rc = FLCT_HIP_OK;
-  if (bf16) {
+  if (big) {
+    BigArgs ba;
+    ba.f1 = f1;
+    ba.f2 = f2;
+    ba.nx = nx;
+    ba.ny = ny;
+    ba.nsize = pl.r.nsize;
+    ba.N = N;
+    ba.R = big_rows(tb.big_fwd.logm);
+    ba.sigminv = ta.sigminv;
+    ba.deltinv = ta.deltinv;
+    ba.deltas = ta.deltas;
+    ba.gw = tb.gw;
+    ba.mean1 = sc.mean1;
+    ba.mean2 = sc.mean2;
+    ba.filt = tb.filt;
+    ba.list = ta.list;
+    ba.nwork = npix;
+    ba.nwork_dev = ta.nwork_dev;
+    ba.bias = ta.bias;
+    ba.hires = ta.hires;
+    ba.vx = vx;
+    ba.vy = vy;
+    ba.vm = vm;
+    ba.plane = (double2*)sc.planes;
+    ba.surf = (double*)((char*)sc.planes + (size_t)pl.big_grid * 16 * (size_t)N * N);
+    ba.fwd = tb.big_fwd;
+    ba.inv = tb.big_inv;
+    if ((rc = launch_track_big(lc, ba, pl.big_grid))) return rc;
+  } else if (bf16) {
     if (pl.seg) {
       long long nedge = 0;
       if ((rc = launch_edge_list(lc, ta.list, ta.nwork_dev, nx, ny, ta.nsize, sc.elist, counts + 2,
@@ -525,7 +578,9 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
     if ((rc = (M <= 12) ? dispatch_track_f32_lo(M, lc, ta) : dispatch_track_f32_hi(M, lc, ta)))
       return rc;
   }
-  if (refine) {
+  if (big) {
+    // tracked above
+  } else if (refine) {
     if (ev) HIPCHK(hipEventRecord(ev[4], s));
     // FP64 re-evaluation of the flagged pixels (count read on the device)
     TrackArgs tr = ta;
@@ -576,10 +631,11 @@ static double ev_ms(hipEvent_t a, hipEvent_t b) {
 
 // Scratch buffers of the context sized for a plan
 static int ensure_scratch(flct_hip_ctx* c, const Plan& pl, Scratch* sc) {
-  size_t sz[7];
+  size_t sz[kNumScratch];
   scratch_sizes(pl, sz);
-  DevBuf* b[7] = {&c->list, &c->hs1, &c->hs2, &c->mean1, &c->mean2, &c->flags, &c->elist};
-  for (int k = 0; k < 7; ++k)
+  DevBuf* b[kNumScratch] = {&c->list, &c->hs1,   &c->hs2,   &c->mean1,
+                            &c->mean2, &c->flags, &c->elist, &c->bigp};
+  for (int k = 0; k < kNumScratch; ++k)
     if (int rc = ensure(*b[k], sz[k] ? sz[k] : 256)) return rc;
   sc->list = (int*)c->list.p;
   sc->hs1 = (double*)c->hs1.p;
@@ -588,10 +644,31 @@ static int ensure_scratch(flct_hip_ctx* c, const Plan& pl, Scratch* sc) {
   sc->mean2 = (double*)c->mean2.p;
   sc->flags = (int*)c->flags.p;
   sc->elist = (int*)c->elist.p;
+  sc->planes = c->bigp.p;
   return FLCT_HIP_OK;
 }
 
 static void root_table(std::vector<double>& t, int n);
+static int dft_tables(flct_hip_ctx* c, int n, int sign, const flct_hip_ctx::DftTab** out);
+
+// transform tables of a box above 96 in both directions (cached in the
+// context; the fft_rows.hpp layout: twiddles, chirp, chirp transform)
+static int big_tables(flct_hip_ctx* c, int N, Tables* tb) {
+  for (int sign : {-1, +1}) {
+    const flct_hip_ctx::DftTab* t = nullptr;
+    if (int rc = dft_tables(c, N, sign, &t)) return rc;
+    BigTab& b = (sign < 0) ? tb->big_fwd : tb->big_inv;
+    const int M = 1 << t->logm;
+    const double2* base = (const double2*)t->buf.p;
+    b.logm = t->logm;
+    b.bluestein = t->bluestein;
+    b.sign = sign;
+    b.tw = base;
+    b.chirp = base + M / 2;
+    b.bhat = base + M / 2 + N;
+  }
+  return FLCT_HIP_OK;
+}
 static int row_dft(flct_hip_ctx* c, const double* in, long long irs, long long ies, bool real_in,
                    int n, int nout, int nrows, double* out, long long ors, long long oes, int sign,
                    const double* tw_direct);
@@ -771,6 +848,7 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
     }
     tb.segtab = c->segtab.p;
   }
+  if (pl.path == PATH_FP64_PLANE && (rc = big_tables(c, pl.N, &tb))) return rc;
   Scratch sc;
   if ((rc = ensure_scratch(c, pl, &sc))) return rc;
   if ((rc = ensure(c->count, 4 * sizeof(unsigned int))) ||
@@ -785,7 +863,7 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
   unsigned int hc[4];
   HIPCHK(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  const bool refine = (pl.path != PATH_FP64);
+  const bool refine = (pl.path != PATH_FP64 && pl.path != PATH_FP64_PLANE);
   c->last_path = pl.path;
   c->last_tracked = pl.sparse ? (long long)hc[0] : (long long)nx * ny;
   c->refined = refine ? (long long)hc[1] : 0;
@@ -842,7 +920,7 @@ extern "C" const char* flct_hip_last_path(flct_hip_ctx* c) {
 }
 
 extern "C" int flct_hip_set_path(flct_hip_ctx* c, int precision, int nbox, const char* path) {
-  if (!c || nbox < 4 || nbox % 4 != 0 || nbox / 4 > kTrackMaxM) return FLCT_HIP_E_ARG;
+  if (!c || nbox < 4 || nbox % 4 != 0 || nbox > kBigMaxN) return FLCT_HIP_E_ARG;
   if (precision < FLCT_HIP_FP64 || precision > FLCT_HIP_SPLIT_BF16) return FLCT_HIP_E_ARG;
   // one choice for calls with and without -b (a -b call the row-segment
   // kernel cannot serve runs the per-pixel one)
@@ -854,7 +932,7 @@ extern "C" int flct_hip_set_path(flct_hip_ctx* c, int precision, int nbox, const
     return FLCT_HIP_OK;
   }
   int p = -1;
-  for (int q = PATH_FP64; q <= PATH_BF16_PIXEL; ++q)
+  for (int q = PATH_FP64; q <= PATH_FP64_PLANE; ++q)
     if (std::strcmp(path, path_name(q)) == 0) p = q;
   if (p < 0) return FLCT_HIP_E_ARG;
   // the path must meet the form's error contract; the row-segment kernel
@@ -971,15 +1049,20 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
     }
   }
   // scratch for the largest pair, counters per pair
-  size_t need[7] = {256, 256, 256, 256, 256, 256, 256};
+  size_t need[kNumScratch];
+  for (int i = 0; i < kNumScratch; ++i) need[i] = 256;
   for (int k = 0; k < npairs; ++k) {
-    size_t sz[7];
+    size_t sz[kNumScratch];
     scratch_sizes(pl[k], sz);
-    for (int i = 0; i < 7; ++i) need[i] = std::max(need[i], sz[i]);
+    for (int i = 0; i < kNumScratch; ++i) need[i] = std::max(need[i], sz[i]);
   }
   int rc;
-  for (int i = 0; i < 7; ++i)
+  for (int i = 0; i < kNumScratch; ++i)
     if ((rc = ensure_b(c, c->bscr[i], need[i]))) return rc;
+  // transform tables of boxes above 96 (cached in the context before capture)
+  std::vector<Tables> btab(npairs);
+  for (int k = 0; k < npairs; ++k)
+    if (pl[k].path == PATH_FP64_PLANE && (rc = big_tables(c, pl[k].N, &btab[k]))) return rc;
   if ((rc = ensure_b(c, c->barena, arena.size())) ||
       (rc = ensure_b(c, c->bcount, (size_t)npairs * 32)))
     return rc;
@@ -1004,12 +1087,13 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
     sc.mean2 = (double*)c->bscr[4].p;
     sc.flags = (int*)c->bscr[5].p;
     sc.elist = (int*)c->bscr[6].p;
+    sc.planes = c->bscr[7].p;
     unsigned char* base = (unsigned char*)c->barena.p;
     LaunchCtx lc{s, c->num_cu, c->lds_per_block};
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     int erc = FLCT_HIP_OK;
     for (int k = 0; k < npairs && !erc; ++k) {
-      Tables tb;
+      Tables tb = btab[k];
       tb.gw = (const double*)(base + o_gw[k]);
       tb.filt = (o_filt[k] != (size_t)-1) ? (const double*)(base + o_filt[k]) : nullptr;
       tb.segtab = (o_seg[k] != (size_t)-1) ? (const void*)(base + o_seg[k]) : nullptr;
@@ -1041,7 +1125,8 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
   for (int k = 0; k < npairs; ++k) {
     const unsigned int* ck = hc.data() + (size_t)k * 8;
     pairs[k].tracked = pl[k].sparse ? (long long)ck[0] : (long long)pl[k].nx * pl[k].ny;
-    pairs[k].refined = (pl[k].path != PATH_FP64) ? (long long)ck[1] : 0;
+    pairs[k].refined =
+        (pl[k].path != PATH_FP64 && pl[k].path != PATH_FP64_PLANE) ? (long long)ck[1] : 0;
     tracked += pairs[k].tracked;
     refined += pairs[k].refined;
   }

@@ -60,8 +60,13 @@ def test_option_checks_match_reference_exit_sites():
     assert _check(7, 9, deltat=1, deltas=1, sigma=0, skip=3)[0] == 0
     assert _check(64, 64, deltat=1, deltas=1, sigma=-3) == (0, 0)
     assert _check(64, 64, deltat=1, deltas=1, sigma=0, skip=64)[0] == L.FLCT_HIP_E_SKIP_TOO_BIG
-    # box sizes beyond 96 are not instantiated
-    assert _check(400, 400, deltat=1, deltas=1, sigma=30)[0] == L.FLCT_HIP_E_UNSUPPORTED
+    # boxes above 96 run the plane kernel up to 1024 (sigma 24 gives 100,
+    # 30 gives 128, 50 gives 212); larger boxes are refused
+    assert _check(400, 400, deltat=1, deltas=1, sigma=24) == (0, 100)
+    assert _check(400, 400, deltat=1, deltas=1, sigma=30) == (0, 128)
+    assert _check(400, 400, deltat=1, deltas=1, sigma=50) == (0, 212)
+    assert _check(4000, 4000, deltat=1, deltas=1, sigma=239) == (0, 1024)
+    assert _check(4000, 4000, deltat=1, deltas=1, sigma=240)[0] == L.FLCT_HIP_E_UNSUPPORTED  # 1028
 
 
 def test_skip_and_threshold_grammar():
