@@ -76,7 +76,8 @@ def test_big_box_threshold_and_skip_fill(ctx):
 
 
 def test_big_box_every_form_runs_fp64(ctx):
-    a, b = _pair(140, 150, 10)
+    a, b = _pair(170, 180, 10)
+    assert ctx.box_size(170, 180, 24.0) == 100
     ref = ctx.track(a, b, 1.0, 1.0, 24.0)
     for form in ("fp32", "split_bf16"):
         g = ctx.track(a, b, 1.0, 1.0, 24.0, precision=form)
