@@ -30,6 +30,8 @@ extern "C" {
 #define FLCT_HIP_E_ALLOC 8         /* device allocation failed */
 #define FLCT_HIP_E_HIP 9           /* a HIP runtime call failed */
 #define FLCT_HIP_E_SHAPE 10        /* warp.c:182-188 "bad shift arrays" */
+#define FLCT_HIP_E_NOSCRATCH 11    /* flct_hip_track_device_noalloc: the call needs
+                                      more than its 256 MiB device array */
 
 /* arithmetic forms of the tracker */
 #define FLCT_HIP_FP64 0        /* plain form: FP64 throughout (drop-in) */
@@ -101,6 +103,23 @@ int flct_hip_track(flct_hip_ctx *ctx, const double *f1, const double *f2, int nx
 int flct_hip_track_device(flct_hip_ctx *ctx, const double *d_f1, const double *d_f2, int nx,
                           int ny, const flct_hip_opts *opts, double *d_vx, double *d_vy,
                           double *d_vm);
+
+/* flct_hip_track_device without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every table and scratch buffer of the call (box means and their row sums,
+ * pixel lists, counters, transform tables, the planes of boxes above 96)
+ * lives in one static 256 MiB device array that each device holds from
+ * library load until the process ends; the caller passes only the inputs
+ * and outputs.  A call whose scratch exceeds the array returns
+ * FLCT_HIP_E_NOSCRATCH (every pixel tracked in FP64 fits up to about
+ * 2900 x 2900 pixels); sigma <= 0 returns FLCT_HIP_E_UNSUPPORTED.  All
+ * calls on one device share the array: calls that overlap on one device,
+ * from any context or thread, race.  Results equal flct_hip_track_device's
+ * for the same inputs and options. */
+int flct_hip_track_device_noalloc(flct_hip_ctx *ctx, const double *d_f1, const double *d_f2,
+                                  int nx, int ny, const flct_hip_opts *opts, double *d_vx,
+                                  double *d_vy, double *d_vm);
 
 /* One image pair of a batch: device arrays (C-space, nx*ny doubles) and the
  * options of its own flct run; tracked, refined and status are outputs. */

@@ -26,6 +26,7 @@ FLCT_HIP_E_NODEVICE = 7
 FLCT_HIP_E_ALLOC = 8
 FLCT_HIP_E_HIP = 9
 FLCT_HIP_E_SHAPE = 10
+FLCT_HIP_E_NOSCRATCH = 11
 
 FP64 = 0
 FP32 = 1
@@ -41,6 +42,7 @@ EXPORTS = (
     "flct_hip_destroy",
     "flct_hip_track",
     "flct_hip_track_device",
+    "flct_hip_track_device_noalloc",
     "flct_hip_track_batch",
     "flct_hip_last_stats",
     "flct_hip_last_refined",
@@ -125,6 +127,8 @@ def lib():
         L.flct_hip_destroy.argtypes = [vp]
         L.flct_hip_track.argtypes = [vp, _dp, _dp, i, i, ctypes.POINTER(Opts), _dp, _dp, _dp]
         L.flct_hip_track_device.argtypes = [vp, vp, vp, i, i, ctypes.POINTER(Opts), vp, vp, vp]
+        L.flct_hip_track_device_noalloc.argtypes = [vp, vp, vp, i, i, ctypes.POINTER(Opts), vp, vp,
+                                                    vp]
         L.flct_hip_track_batch.argtypes = [vp, ctypes.POINTER(Pair), i]
         L.flct_hip_last_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong),
                                           ctypes.POINTER(d), ctypes.POINTER(d)]

@@ -161,6 +161,13 @@ class Context:
         check(_lib.lib().flct_hip_track_device(self._h, d_f1, d_f2, nx, ny, ctypes.byref(opts),
                                                d_vx, d_vy, d_vm), "flct_hip_track_device")
 
+    def track_device_noalloc(self, d_f1, d_f2, nx, ny, d_vx, d_vy, d_vm, opts):
+        """flct_hip_track_device without device allocation: its scratch is the
+        library's static 256 MiB device array (FLCT_HIP_E_NOSCRATCH beyond)."""
+        check(_lib.lib().flct_hip_track_device_noalloc(self._h, d_f1, d_f2, nx, ny,
+                                                       ctypes.byref(opts), d_vx, d_vy, d_vm),
+              "flct_hip_track_device_noalloc")
+
     def track_batch(self, pairs, **common):
         """Track several image pairs in one graph launch (flct_hip_track_batch).
 

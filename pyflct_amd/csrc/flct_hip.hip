@@ -25,6 +25,15 @@
 using namespace flct;
 
 
+// Device memory of flct_hip_track_device_noalloc: every table and scratch
+// buffer of one such call is carved from this array, which each device
+// holds once for the life of the process.  In host code the name only
+// stands for the symbol; its address on a device comes from
+// hipGetSymbolAddress (flct_hip_create) and is the only form passed to
+// kernels and copies.
+constexpr size_t kNoAllocBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 namespace {
 
 
@@ -70,6 +79,7 @@ struct flct_hip_ctx {
   double last_main_ms = 0.;    // the tracking kernel alone
   double last_refine_ms = 0.;  // FP64 re-evaluation of flagged pixels (split-BF16, FP32)
   int nu_nx = 0, nu_ny = 0;  // image size of the NUFFT tables in nuTab
+  unsigned char* noalloc = nullptr;  // this device's address of `scratch`
   int warp_method = FLCT_HIP_WARP_AUTO;
   int last_warp_method = 0;
 };
@@ -111,6 +121,7 @@ extern "C" const char* flct_hip_strerror(int s) {
     case FLCT_HIP_E_ALLOC: return "device allocation failed";
     case FLCT_HIP_E_HIP: return "HIP runtime error";
     case FLCT_HIP_E_SHAPE: return "bad shift arrays";
+    case FLCT_HIP_E_NOSCRATCH: return "problem needs more than the 256 MiB no-allocation scratch";
     default: return "unknown status";
   }
 }
@@ -136,6 +147,15 @@ extern "C" int flct_hip_create(flct_hip_ctx** out, int device) {
       delete c;
       return FLCT_HIP_E_HIP;
     }
+  }
+  // the no-allocation entry point's array on this device (set-up, once per
+  // context, with the context's device current)
+  void* sym = nullptr;
+  if (hipGetSymbolAddress(&sym, HIP_SYMBOL(scratch)) == hipSuccess && sym) {
+    c->noalloc = static_cast<unsigned char*>(sym);
+  } else {
+    c->noalloc = nullptr;  // flct_hip_track_device_noalloc then returns FLCT_HIP_E_HIP
+    (void)hipGetLastError();
   }
   // kernel choice of the split-BF16 form: "pixel" keeps every pixel on the
   // per-pixel kernels where they meet the form's contract (boxes up to
@@ -802,6 +822,8 @@ static int track_whole(flct_hip_ctx* c, const double* f1, const double* f2, int 
   return FLCT_HIP_OK;
 }
 
+static int end_call(flct_hip_ctx* c, const Plan& pl, const unsigned int* counts);
+
 extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const double* f2, int nx,
                                      int ny, const flct_hip_opts* o, double* vx, double* vy,
                                      double* vm) {
@@ -859,6 +881,14 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
   if ((rc = enqueue_track(lc, pl, tb, sc, counts, (unsigned long long*)c->absmax.p, f1, f2, vx, vy,
                           vm, c->ev)))
     return rc;
+  return end_call(c, pl, counts);
+}
+
+// End of a single-pair call: waits for the stream, reads the counters and
+// keeps the call's statistics in the context.
+static int end_call(flct_hip_ctx* c, const Plan& pl, const unsigned int* counts) {
+  hipStream_t s = c->stream;
+  const int nx = pl.nx, ny = pl.ny;
   HIPCHK(hipEventRecord(c->ev[3], s));
   unsigned int hc[4];
   HIPCHK(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, s));
@@ -877,6 +907,114 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
     c->last_refine_ms = 0.;
   }
   return FLCT_HIP_OK;
+}
+
+static int dft_table_host(int n, int sign, std::vector<double>& h, int* logm, int* bluestein);
+
+// flct_hip_track_device with every table and scratch buffer carved from the
+// static `scratch` array of the device (its address from flct_hip_create):
+// no device allocation.  The tables are copied from host memory on the
+// stream; the counters are cleared by kernels (enqueue_track).  Boxes above
+// 96 run k_track_big with as many blocks as their planes leave room for.
+extern "C" int flct_hip_track_device_noalloc(flct_hip_ctx* c, const double* f1, const double* f2,
+                                             int nx, int ny, const flct_hip_opts* o, double* vx,
+                                             double* vy, double* vm) {
+  if (!c || !f1 || !f2 || !o || !vx || !vy || !vm || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
+  {
+    Resolved r0;
+    const int rc0 = resolve(o, nx, ny, &r0);
+    if (rc0) return rc0;
+    if (r0.whole) return FLCT_HIP_E_UNSUPPORTED;  // sigma <= 0: flct_hip_track_device
+  }
+  if (!c->noalloc) return FLCT_HIP_E_HIP;
+  Plan pl;
+  int rc = make_plan(nx, ny, o, c, &pl, c->force_path);
+  if (rc) return rc;
+  const int N = pl.N;
+  const long long npix = (long long)nx * ny;
+  // host tables
+  std::vector<unsigned short> segt;
+  if (pl.seg) seg_tables(N, o->sigma, segt);
+  std::vector<double> bt[2];
+  int blog[2] = {0, 0}, bblue[2] = {0, 0};
+  const bool big = (pl.path == PATH_FP64_PLANE);
+  if (big)
+    for (int k = 0; k < 2; ++k)
+      if ((rc = dft_table_host(N, k ? +1 : -1, bt[k], &blog[k], &bblue[k]))) return rc;
+  // layout of the call in the array, 256-byte aligned pieces
+  size_t off = 0;
+  auto take = [&off](size_t bytes) {
+    const size_t at = off;
+    off += (bytes + 255) / 256 * 256;
+    return at;
+  };
+  const size_t o_cnt = take(4 * sizeof(unsigned int)), o_max = take(2 * sizeof(unsigned long long));
+  const size_t o_gw = take(pl.gw.size() * 8);
+  const size_t o_filt = pl.filt.empty() ? 0 : take(pl.filt.size() * 8);
+  const size_t o_seg = segt.empty() ? 0 : take(segt.size() * 2);
+  const size_t o_bt0 = big ? take(bt[0].size() * 8) : 0, o_bt1 = big ? take(bt[1].size() * 8) : 0;
+  size_t sz[kNumScratch];
+  if (big) {
+    // the planes take what is left: fewer blocks when they do not all fit
+    pl.big_grid = 0;
+    scratch_sizes(pl, sz);
+    size_t fixed = off;
+    for (int k = 0; k < kNumScratch; ++k) fixed += (sz[k] + 255) / 256 * 256 + 256;
+    const size_t plane = 24 * (size_t)N * N;
+    const long long fit = (fixed < kNoAllocBytes) ? (long long)((kNoAllocBytes - fixed) / plane) : 0;
+    if (fit < 1) return FLCT_HIP_E_NOSCRATCH;
+    pl.big_grid = std::min(big_grid(c->num_cu, N, npix), fit);
+  }
+  scratch_sizes(pl, sz);
+  size_t o_sc[kNumScratch];
+  for (int k = 0; k < kNumScratch; ++k) o_sc[k] = take(sz[k] ? sz[k] : 256);
+  if (off > kNoAllocBytes) return FLCT_HIP_E_NOSCRATCH;
+
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  unsigned char* base = c->noalloc;
+  HIPCHK(hipEventRecord(c->ev[0], s));
+  Tables tb;
+  HIPCHK(hipMemcpyAsync(base + o_gw, pl.gw.data(), pl.gw.size() * 8, hipMemcpyHostToDevice, s));
+  tb.gw = (const double*)(base + o_gw);
+  if (!pl.filt.empty()) {
+    HIPCHK(hipMemcpyAsync(base + o_filt, pl.filt.data(), pl.filt.size() * 8, hipMemcpyHostToDevice, s));
+    tb.filt = (const double*)(base + o_filt);
+  }
+  if (!segt.empty()) {
+    HIPCHK(hipMemcpyAsync(base + o_seg, segt.data(), segt.size() * 2, hipMemcpyHostToDevice, s));
+    tb.segtab = base + o_seg;
+  }
+  if (big) {
+    const size_t ob[2] = {o_bt0, o_bt1};
+    for (int k = 0; k < 2; ++k) {
+      HIPCHK(hipMemcpyAsync(base + ob[k], bt[k].data(), bt[k].size() * 8, hipMemcpyHostToDevice, s));
+      BigTab& b = k ? tb.big_inv : tb.big_fwd;
+      const int M = 1 << blog[k];
+      const double2* t = (const double2*)(base + ob[k]);
+      b.logm = blog[k];
+      b.bluestein = bblue[k];
+      b.sign = k ? +1 : -1;
+      b.tw = t;
+      b.chirp = t + M / 2;
+      b.bhat = t + M / 2 + N;
+    }
+  }
+  Scratch sc;
+  sc.list = (int*)(base + o_sc[0]);
+  sc.hs1 = (double*)(base + o_sc[1]);
+  sc.hs2 = (double*)(base + o_sc[2]);
+  sc.mean1 = (double*)(base + o_sc[3]);
+  sc.mean2 = (double*)(base + o_sc[4]);
+  sc.flags = (int*)(base + o_sc[5]);
+  sc.elist = (int*)(base + o_sc[6]);
+  sc.planes = base + o_sc[7];
+  unsigned int* counts = (unsigned int*)(base + o_cnt);
+  LaunchCtx lc{s, c->num_cu, c->lds_per_block};
+  if ((rc = enqueue_track(lc, pl, tb, sc, counts, (unsigned long long*)(base + o_max), f1, f2, vx,
+                          vy, vm, c->ev)))
+    return rc;
+  return end_call(c, pl, counts);
 }
 
 extern "C" int flct_hip_track(flct_hip_ctx* c, const double* f1, const double* f2, int nx, int ny,
@@ -1176,29 +1314,24 @@ static void host_fft(std::vector<double>& re, std::vector<double>& im, int logm)
   }
 }
 
-// tables of the length-n row transform with the given sign (cached)
-static int dft_tables(flct_hip_ctx* c, int n, int sign, const flct_hip_ctx::DftTab** out) {
-  for (const auto& t : c->dft_tabs)
-    if (t.n == n && t.sign == sign) {
-      *out = &t;
-      return FLCT_HIP_OK;
-    }
-  flct_hip_ctx::DftTab t;
-  t.n = n;
-  t.sign = sign;
-  t.bluestein = (n & (n - 1)) != 0;
+// host tables of the length-n row transform with the given sign (the
+// fft_rows.hpp layout: twiddles, then chirp and chirp transform for
+// Bluestein), as double2 entries
+static int dft_table_host(int n, int sign, std::vector<double>& h, int* logm, int* bluestein) {
+  const int blue = (n & (n - 1)) != 0;
   int l = 0;
-  while ((1 << l) < (t.bluestein ? 2 * n - 1 : n)) ++l;
-  t.logm = l;
+  while ((1 << l) < (blue ? 2 * n - 1 : n)) ++l;
   const int M = 1 << l;
   if (l > kRowDftMaxLog2) return FLCT_HIP_E_UNSUPPORTED;
-  std::vector<double> h;  // double2 entries
+  *logm = l;
+  *bluestein = blue;
+  h.clear();
   h.reserve(2 * (M / 2 + n + M));
   for (int e = 0; e < M / 2; ++e) {
     h.push_back(std::cos(2. * M_PI * e / M));
     h.push_back(std::sin(2. * M_PI * e / M));
   }
-  if (t.bluestein) {
+  if (blue) {
     // w_m = e^{sign pi i m^2 / n}, the exponent reduced mod 2n exactly
     std::vector<double> wr(n), wi(n);
     for (int m = 0; m < n; ++m) {
@@ -1225,7 +1358,22 @@ static int dft_tables(flct_hip_ctx* c, int n, int sign, const flct_hip_ctx::DftT
       h.push_back(bi[k] / M);
     }
   }
+  return FLCT_HIP_OK;
+}
+
+// tables of the length-n row transform with the given sign (cached)
+static int dft_tables(flct_hip_ctx* c, int n, int sign, const flct_hip_ctx::DftTab** out) {
+  for (const auto& t : c->dft_tabs)
+    if (t.n == n && t.sign == sign) {
+      *out = &t;
+      return FLCT_HIP_OK;
+    }
+  flct_hip_ctx::DftTab t;
+  t.n = n;
+  t.sign = sign;
+  std::vector<double> h;
   int rc;
+  if ((rc = dft_table_host(n, sign, h, &t.logm, &t.bluestein))) return rc;
   if ((rc = ensure(t.buf, h.size() * 8))) return rc;
   HIPCHK(hipMemcpy(t.buf.p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
   c->dft_tabs.push_back(t);
