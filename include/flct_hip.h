@@ -156,8 +156,11 @@ int flct_hip_last_stats(flct_hip_ctx *ctx, long long *tracked, double *kernel_ms
 /* Number of pixels the last FLCT_HIP_SPLIT_BF16 or FLCT_HIP_FP32 call
  * re-evaluated in FP64 because its correlation surface could not settle them
  * (a second cell within 1e-4 (split-BF16) or 1e-5 (FP32) of the maximum, or
- * an ill-conditioned peak fit); 0 for FLCT_HIP_FP64.  No counterpart in the
- * reference, which has one precision. */
+ * an ill-conditioned peak fit).  The FP64 row-shared path ("fp64-rowseg")
+ * counts the pixels it hands to the quad-FFT kernels: boxes constant at an
+ * inexact mean, or whose structure is below 1e-4 of their distance from the
+ * segment's reference mean; the quad-FFT FP64 path reports 0.  No
+ * counterpart in the reference, which has one precision. */
 int flct_hip_last_refined(flct_hip_ctx *ctx, long long *refined);
 
 /* Device time in milliseconds of the last track call's tracking kernel alone

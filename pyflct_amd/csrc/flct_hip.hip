@@ -21,6 +21,7 @@
 #include "fft_rows.hpp"
 #include "track_launch.hpp"
 #include "track_big.hpp"
+#include "track_rs.hpp"
 
 using namespace flct;
 
@@ -277,7 +278,8 @@ enum Path {
   PATH_FP32 = 1,
   PATH_BF16_SEG = 2,
   PATH_BF16_PIXEL = 3,
-  PATH_FP64_PLANE = 4
+  PATH_FP64_PLANE = 4,
+  PATH_FP64_ROWSEG = 5  // FP64, shared row transforms (k_track_rs)
 };
 constexpr int kQuadMaxN = 4 * kTrackMaxM;  // largest box of the quad-FFT kernels
 
@@ -288,6 +290,7 @@ static const char* path_name(int p) {
     case PATH_BF16_SEG: return "bf16-segment";
     case PATH_BF16_PIXEL: return "bf16-pixel";
     case PATH_FP64_PLANE: return "fp64-plane";
+    case PATH_FP64_ROWSEG: return "fp64-rowseg";
     default: return "none";
   }
 }
@@ -302,20 +305,28 @@ constexpr int kBf16PixelMaxBox = 72;
 
 // candidate paths of a form and box size (the first is the untuned default
 // unless default_path says otherwise)
+// The row-shared FP64 kernel serves calls without a skip lattice, -b or -h.
+static bool rowseg_ok(int N, int bias, int hires, int skipon) {
+  return N <= kQuadMaxN && !bias && !hires && !skipon;
+}
+
 static std::vector<int> path_candidates(int precision, int N, int nx, int ny, int bias,
-                                        int hires) {
+                                        int hires, int skipon = 0) {
   if (N > kQuadMaxN) return {PATH_FP64_PLANE};  // boxes above 96: FP64 in every form
   if (hires) return {PATH_FP64};  // the hires refinement runs in the FP64 kernels
-  if (precision == FLCT_HIP_FP32) return {PATH_FP32, PATH_FP64};
-  if (precision == FLCT_HIP_SPLIT_BF16) {
-    std::vector<int> v;
+  std::vector<int> v;
+  if (precision == FLCT_HIP_FP32) {
+    v = {PATH_FP32, PATH_FP64};
+  } else if (precision == FLCT_HIP_SPLIT_BF16) {
     if (seg_supported(N, nx, ny, bias)) v.push_back(PATH_BF16_SEG);
     if (N <= kBf16PixelMaxBox) v.push_back(PATH_BF16_PIXEL);
     v.push_back(PATH_FP32);
     v.push_back(PATH_FP64);
-    return v;
+  } else {
+    v = {PATH_FP64};
   }
-  return {PATH_FP64};
+  if (rowseg_ok(N, bias, hires, skipon)) v.push_back(PATH_FP64_ROWSEG);
+  return v;
 }
 
 // Untuned choice: the split-BF16 request on boxes up to 20 runs the FP32
@@ -325,6 +336,22 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
 // within the split-BF16 error bound and measured 29-41 % faster than the
 // FP64 kernels and 2.9-3.7x faster than the dense-DFT BF16 kernel there
 // (2048^2, boxes 68..84).
+// Box sizes where the row-shared FP64 kernel was measured faster than
+// k_track by 4 % or more (2048^2 full-disk pair, every pixel tracked, device
+// ms k_track / k_track_rs): 8 3.03/2.60, 12 5.85/4.99, 32 31.1/29.7,
+// 36 56.2/42.9, 40 80.5/76.3, 48 125/114, 52 174/150, 60 202/172, 64 211/153,
+// 68 448/407, 72 443/356, 96 700/583.  Level or slower elsewhere (4, 16..28,
+// 44, 76..92).
+static bool rowseg_default(int N) {
+  switch (N) {
+    case 8: case 12: case 32: case 36: case 40: case 48: case 52: case 60: case 64: case 68:
+    case 72: case 96:
+      return true;
+    default:
+      return false;
+  }
+}
+
 static int default_path(int precision, int N, int nx, int ny, int bias, int bf16_kernel) {
   if (N > kQuadMaxN) return PATH_FP64_PLANE;
   if (precision == FLCT_HIP_FP32) return PATH_FP32;
@@ -351,6 +378,8 @@ struct Plan {
   int path = PATH_FP64;   // kernel path
   int ixc = 0, iyc = 0;   // skip-fill lattice size
   long long big_grid = 0; // blocks of k_track_big (boxes above 96)
+  int rs_nband = 0;       // k_track_rs: column bands of at most kRsSeg pixels
+  long long rs_grid = 0;  // k_track_rs: blocks
   std::vector<double> gw, filt;
 };
 
@@ -406,10 +435,25 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Pla
     pl->path = force_path;
   } else {
     const auto it = c->tuned.find(std::make_tuple(o->precision, N, o->bias ? 1 : 0));
-    pl->path = (it != c->tuned.end()) ? it->second
-                                      : default_path(o->precision, N, nx, ny, o->bias, c->bf16_kernel);
+    if (it != c->tuned.end()) {
+      pl->path = it->second;
+    } else {
+      pl->path = default_path(o->precision, N, nx, ny, o->bias, c->bf16_kernel);
+      // FP64 without a threshold: the row-shared kernel where it was faster
+      if (pl->path == PATH_FP64 && o->thresh == 0. && rowseg_default(N) &&
+          rowseg_ok(N, o->bias, o->hires, pl->r.skipon))
+        pl->path = PATH_FP64_ROWSEG;
+    }
   }
   if (pl->path == PATH_BF16_SEG && !seg_supported(N, nx, ny, o->bias)) pl->path = PATH_BF16_PIXEL;
+  if (pl->path == PATH_FP64_ROWSEG) {
+    if (rowseg_ok(N, o->bias, o->hires, pl->r.skipon)) {
+      pl->rs_nband = (ny + kRsSeg - 1) / kRsSeg;
+      pl->rs_grid = rs_grid(N / 4, c->num_cu, (long long)nx * pl->rs_nband);
+    } else {
+      pl->path = PATH_FP64;
+    }
+  }
   if (o->hires) pl->path = PATH_FP64;  // -h: the FP64 kernels carry the hires refinement
   if (N > kQuadMaxN) {
     // boxes above 96 run k_track_big in every form, -h and -b included
@@ -462,7 +506,9 @@ void scratch_sizes(const Plan& pl, size_t sz[kNumScratch]) {
   sz[3] = sz[4] = (size_t)nrx * nsx * 8;
   sz[5] = (pl.path != PATH_FP64) ? npix * sizeof(int) : 0;
   sz[6] = pl.seg ? (pl.sparse ? npix : (long long)pl.nx * (N - 1)) * sizeof(int) : 0;
-  sz[7] = (pl.path == PATH_FP64_PLANE) ? (size_t)pl.big_grid * 24 * (size_t)N * N : 0;
+  sz[7] = (pl.path == PATH_FP64_PLANE)    ? (size_t)pl.big_grid * 24 * (size_t)N * N
+          : (pl.path == PATH_FP64_ROWSEG) ? (size_t)pl.rs_grid * 16 * N * (size_t)(kRsSeg + N)
+                                          : 0;
 }
 
 // Enqueues the tracking pipeline of one pair on lc.stream: no host
@@ -541,7 +587,7 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
   ta.absmax = bf16 ? absmax : nullptr;
   // the split-BF16 and FP32 kernels list the pixels their surface cannot
   // settle; those are tracked again by the FP64 kernels
-  const bool refine = bf16 || pl.path == PATH_FP32;
+  const bool refine = bf16 || pl.path == PATH_FP32 || pl.path == PATH_FP64_ROWSEG;
   ta.flag_list = refine ? sc.flags : nullptr;
   ta.flag_count = refine ? counts + 1 : nullptr;
   if (ev) HIPCHK(hipEventRecord(ev[1], s));
@@ -575,6 +621,16 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
     ba.fwd = tb.big_fwd;
     ba.inv = tb.big_inv;
     if ((rc = launch_track_big(lc, ba, pl.big_grid))) return rc;
+  } else if (pl.path == PATH_FP64_ROWSEG) {
+    RsArgs ra;
+    ra.t = ta;
+    ra.thresh = o->thresh;
+    ra.thresh_rel = pl.relative ? 1 : 0;
+    ra.absmax = absmax;
+    ra.nband = pl.rs_nband;
+    ra.ustride = kRsSeg + N;
+    ra.phi = (double2*)sc.planes;
+    if ((rc = dispatch_track_rs(M, lc, ra, pl.rs_grid))) return rc;
   } else if (bf16) {
     if (pl.seg) {
       long long nedge = 0;
@@ -893,7 +949,8 @@ static int end_call(flct_hip_ctx* c, const Plan& pl, const unsigned int* counts)
   unsigned int hc[4];
   HIPCHK(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  const bool refine = (pl.path != PATH_FP64 && pl.path != PATH_FP64_PLANE);
+  const bool refine = (pl.path == PATH_FP32 || pl.path == PATH_BF16_SEG ||
+                       pl.path == PATH_BF16_PIXEL || pl.path == PATH_FP64_ROWSEG);
   c->last_path = pl.path;
   c->last_tracked = pl.sparse ? (long long)hc[0] : (long long)nx * ny;
   c->refined = refine ? (long long)hc[1] : 0;
@@ -1070,7 +1127,7 @@ extern "C" int flct_hip_set_path(flct_hip_ctx* c, int precision, int nbox, const
     return FLCT_HIP_OK;
   }
   int p = -1;
-  for (int q = PATH_FP64; q <= PATH_FP64_PLANE; ++q)
+  for (int q = PATH_FP64; q <= PATH_FP64_ROWSEG; ++q)
     if (std::strcmp(path, path_name(q)) == 0) p = q;
   if (p < 0) return FLCT_HIP_E_ARG;
   // the path must meet the form's error contract; the row-segment kernel
@@ -1102,7 +1159,7 @@ extern "C" int flct_hip_autotune(flct_hip_ctx* c, const double* f1, const double
     return rc;
   int best = -1;
   double best_ms = 0.;
-  for (int p : path_candidates(o->precision, pl.N, nx, ny, o->bias, o->hires)) {
+  for (int p : path_candidates(o->precision, pl.N, nx, ny, o->bias, o->hires, pl.r.skipon)) {
     c->force_path = p;
     double t = 0.;
     for (int rep = 0; rep < 3 && !rc; ++rep) {  // first run warms up
@@ -1263,8 +1320,11 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
   for (int k = 0; k < npairs; ++k) {
     const unsigned int* ck = hc.data() + (size_t)k * 8;
     pairs[k].tracked = pl[k].sparse ? (long long)ck[0] : (long long)pl[k].nx * pl[k].ny;
+    const int pk = pl[k].path;
     pairs[k].refined =
-        (pl[k].path != PATH_FP64 && pl[k].path != PATH_FP64_PLANE) ? (long long)ck[1] : 0;
+        (pk == PATH_FP32 || pk == PATH_BF16_SEG || pk == PATH_BF16_PIXEL || pk == PATH_FP64_ROWSEG)
+            ? (long long)ck[1]
+            : 0;
     tracked += pairs[k].tracked;
     refined += pairs[k].refined;
   }

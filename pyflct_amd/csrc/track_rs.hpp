@@ -1,0 +1,26 @@
+// FP64 tracker, row-shared form (track_rs.hip): arguments and launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "track_launch.hpp"
+
+namespace flct {
+
+constexpr int kRsSeg = 64;  // pixels per row segment, at most
+
+struct RsArgs {
+  TrackArgs t;  // images, box size, means, windows, filter, outputs
+  double thresh = 0.;  // threshold (flct.c:721-722), 0: off
+  int thresh_rel = 0;  // thresh is relative: times max(max |f1|, max |f2|) from absmax
+  const unsigned long long* absmax = nullptr;
+  int nband = 1;       // column bands of at most kRsSeg pixels
+  int ustride = 0;     // row stride of Phi (complex), kRsSeg + N
+  double2* phi = nullptr;  // per block: N x ustride complex
+};
+
+// blocks of the persistent grid for box size 4*M (0: no such box)
+long long rs_grid(int M, int num_cu, long long nseg);
+int dispatch_track_rs(int M, const LaunchCtx& c, const RsArgs& a, long long grid);
+
+}  // namespace flct

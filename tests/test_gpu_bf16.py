@@ -181,6 +181,7 @@ def test_set_path_rules(ctx):
             ctx.set_path(*args)
     f1, f2, _ = synth.cfg4(n=192)
     ctx.set_path("split_bf16", 64, "fp64-fft")
+    ctx.set_path("fp64", 64, "fp64-fft")
     try:
         got = ctx.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
         assert ctx.last_path() == "fp64-fft"
@@ -189,6 +190,7 @@ def test_set_path_rules(ctx):
             assert np.array_equal(x, y)
     finally:
         ctx.set_path("split_bf16", 64, None)
+        ctx.set_path("fp64", 64, None)
     ctx.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
     assert ctx.last_path() == "bf16-segment"
 
@@ -201,7 +203,7 @@ def test_paths_and_autotune():
     try:
         f1, f2, _ = synth.cfg4(n=384)
         ref = c.track(f1, f2, 1.0, 1.0, 15.0)
-        assert c.last_path() == "fp64-fft"
+        assert c.last_path() == "fp64-rowseg"  # FP64 default at box 64
         c.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
         assert c.last_path() == "bf16-segment"
         b84 = c.track(f1, f2, 1.0, 1.0, 20.0, precision="split_bf16")  # box 84
@@ -215,9 +217,9 @@ def test_paths_and_autotune():
             c.upload(d[0], f1)
             c.upload(d[1], f2)
             for form, sigma, allowed in (("split_bf16", 15.0, {"bf16-segment", "bf16-pixel", "fp32-fft",
-                                                               "fp64-fft"}),
-                                         ("fp32", 9.0, {"fp32-fft", "fp64-fft"}),
-                                         ("fp64", 12.0, {"fp64-fft"})):
+                                                               "fp64-fft", "fp64-rowseg"}),
+                                         ("fp32", 9.0, {"fp32-fft", "fp64-fft", "fp64-rowseg"}),
+                                         ("fp64", 12.0, {"fp64-fft", "fp64-rowseg"})):
                 best = c.autotune(d[0], d[1], 384, 384, c.make_opts(1.0, 1.0, sigma, precision=form))
                 assert best in allowed, (form, best)
                 got = c.track(f1, f2, 1.0, 1.0, sigma, precision=form)

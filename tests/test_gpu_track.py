@@ -34,24 +34,39 @@ SIGMAS_FULL = [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11]          # boxes 4..44
 SIGMAS_SAMPLED = [12, 13, 14, 15, 16, 17, 18, 19, 20, 22, 23]  # 48..96 (no integer sigma gives 56)
 
 
+FP64_PATHS = ["fp64-fft", "fp64-rowseg"]  # quad-FFT k_track; row-shared k_track_rs
+
+
+@pytest.mark.parametrize("path", FP64_PATHS)
 @pytest.mark.parametrize("sigma", SIGMAS_FULL)
-def test_box_sizes_full_image(ctx, sigma):
+def test_box_sizes_full_image(ctx, sigma, path):
     rng = np.random.default_rng(100 + sigma)
     n = 2 * O.box_nsize(sigma, 4096, 4096)
     nx, ny = int(1.6 * n) + 3, int(1.9 * n) + 5  # odd, non-square, edge-clamped boxes
     a, b = smooth_pair(rng, nx, ny)
-    g = ctx.track(a, b, 1.0, 1.0, sigma)
+    ctx.set_path("fp64", n, path)
+    try:
+        g = ctx.track(a, b, 1.0, 1.0, sigma)
+        assert ctx.last_path() == path
+    finally:
+        ctx.set_path("fp64", n, None)
     o = O.flct(a, b, 1.0, 1.0, sigma)
-    assert_parity(g, o, where=f"sigma {sigma} N {n}")
+    assert_parity(g, o, where=f"sigma {sigma} N {n} {path}")
 
 
+@pytest.mark.parametrize("path", FP64_PATHS)
 @pytest.mark.parametrize("sigma", SIGMAS_SAMPLED)
-def test_box_sizes_sampled(ctx, sigma):
+def test_box_sizes_sampled(ctx, sigma, path):
     rng = np.random.default_rng(200 + sigma)
     n = 2 * O.box_nsize(sigma, 4096, 4096)
     nx, ny = int(1.6 * n) + 1, int(1.7 * n) + 2
     a, b = smooth_pair(rng, nx, ny, shift=(1.3, 0.6))
-    g = ctx.track(a, b, 2.0, 0.5, sigma)
+    ctx.set_path("fp64", n, path)
+    try:
+        g = ctx.track(a, b, 2.0, 0.5, sigma)
+        assert ctx.last_path() == path
+    finally:
+        ctx.set_path("fp64", n, None)
     i, j = np.mgrid[0:nx, 0:ny]
     edge = (i < 2) | (i >= nx - 2) | (j < 2) | (j >= ny - 2)
     pix = np.concatenate([np.flatnonzero(edge)[::7], rng.choice(nx * ny, 120, replace=False)])
