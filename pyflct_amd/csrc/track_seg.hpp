@@ -98,30 +98,28 @@ struct SegCfg {
   static constexpr int kTPlane = 32 * RK;
   static constexpr int kTEl = 4 * kTPlane;
   static constexpr int kScratch = 128 * 8 + 64 * 8 + 9 * 8 + 8;  // zx, zp, nb, pad
-  static constexpr int kLdsBytes = kTabEl * 2 + 256 * 4 + kTEl * 2 + 68 * 4 + kWaves * kScratch;
+  static constexpr int kGw = 128;  // window table gw[d + N - 1], 2N - 1 <= 127 doubles
+  static constexpr int kLdsBytes =
+      kTabEl * 2 + 256 * 4 + kTEl * 2 + 68 * 4 + kWaves * kScratch + kGw * 8;
   static_assert(kLdsBytes <= 160 * 1024, "k_track_seg needs more LDS than a CU has");
 };
 
 __device__ __forceinline__ bf16x8 u4_to_bf8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 // 8 consecutive bf16 of an LDS row starting at element e (any alignment);
-// odd is e & 1, uniform over the wave
+// odd is e & 1, uniform over the wave.  Branch-free: five dwords from the
+// one at or below element e, shifted by 2 bytes when e is odd (by 0, the
+// low dword itself, when it is even), so the loads of one tile can be
+// scheduled across the MFMAs of the previous one.
 __device__ __forceinline__ bf16x8 ld8_any(const __bf16* row, int e, bool odd) {
+  const unsigned sh = odd ? 2u : 0u;
+  const unsigned* q = reinterpret_cast<const unsigned*>(row + e - (odd ? 1 : 0));
+  const unsigned d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
   u32x4 r;
-  if (!odd) {
-    const unsigned* q = reinterpret_cast<const unsigned*>(row + e);
-    r[0] = q[0];
-    r[1] = q[1];
-    r[2] = q[2];
-    r[3] = q[3];
-  } else {
-    const unsigned* q = reinterpret_cast<const unsigned*>(row + e - 1);
-    const unsigned d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
-    r[0] = __builtin_amdgcn_alignbyte(d1, d0, 2);
-    r[1] = __builtin_amdgcn_alignbyte(d2, d1, 2);
-    r[2] = __builtin_amdgcn_alignbyte(d3, d2, 2);
-    r[3] = __builtin_amdgcn_alignbyte(d4, d3, 2);
-  }
+  r[0] = __builtin_amdgcn_alignbyte(d1, d0, sh);
+  r[1] = __builtin_amdgcn_alignbyte(d2, d1, sh);
+  r[2] = __builtin_amdgcn_alignbyte(d3, d2, sh);
+  r[3] = __builtin_amdgcn_alignbyte(d4, d3, sh);
   return u4_to_bf8(r);
 }
 
@@ -159,6 +157,8 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
   __bf16* TT = reinterpret_cast<__bf16*>(cmr + 256);
   int* slist = reinterpret_cast<int*>(TT + C::kTEl);
   int* scount = slist + 64;
+  double* gwl = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(slist + 68) +
+                                          C::kWaves * C::kScratch);  // window table
   unsigned char* wsc = reinterpret_cast<unsigned char*>(slist + 68) + wave * C::kScratch;
   float2* zx = reinterpret_cast<float2*>(wsc);  // [2][64] packed-column spectra
   float2* zp = zx + 128;                        // [64] packed cross power
@@ -172,6 +172,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
     u32x4* tz = reinterpret_cast<u32x4*>(TT);
     const u32x4 z = {0u, 0u, 0u, 0u};
     for (int e = tid; e < C::kTEl / 8; e += blockDim.x) tz[e] = z;
+    for (int e = tid; e < 2 * A.t.nsize * 2 - 1; e += blockDim.x) gwl[e] = A.t.gw[e];
   }
 
   const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
@@ -258,23 +259,38 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       const bool ones = (u == 127);
       const int ucl = colok ? u : 0;  // loads stay inside the image
       // rows x = 32 s + 8 g + e of this lane
-      float wr[16];
-      static_for<16>([&](auto q) {
-        const int x = 32 * (q >> 3) + 8 * g1 + (q & 7);
-        wr[int(q)] = (x < N) ? (float)a.gw[(x < N ? x : 0) + imin - i + N - 1] : 0.f;
-      });
-      const __bf16* D1 = A.tab;
+      // Every load of the tile is issued before any arithmetic: the box
+      // column values of both images, the window (from LDS) and the stage-1
+      // table fragments, which serve both images.
+      double fva[2][16];
       static_for<2>([&](auto X) {
         const double* f = (X == 0) ? a.f1 : a.f2;
-        const double mref = (X == 0) ? mref0 : mref1;
-        const float scf = (float)sc[X];
-        bf16x8 Ah[2], Al[2];
-        double fv[16];
-        double csum = 0., wsum = 0.;
         static_for<16>([&](auto q) {
           const int x = 32 * (q >> 3) + 8 * g1 + (q & 7);
           const int xr = (x < N) ? x : 0;
-          fv[int(q)] = f[(long long)(imin + xr) * ny + (cb + ucl)];
+          fva[X][int(q)] = f[(long long)(imin + xr) * ny + (cb + ucl)];
+        });
+      });
+      const __bf16* D1 = A.tab;
+      bf16x8 d1f[2][2][4];  // [kt][s][re_h, re_l, im_h, im_l]
+      static_for<TL>([&](auto kt) {
+        static_for<2>([&](auto s) {
+          const int off = (16 * kt + n1) * C::RS1 + 32 * s + 8 * g1;
+          static_for<4>([&](auto pl) { d1f[kt][s][pl] = ld8(D1 + pl * C::kD1Plane + off); });
+        });
+      });
+      float wr[16];
+      static_for<16>([&](auto q) {
+        const int x = 32 * (q >> 3) + 8 * g1 + (q & 7);
+        wr[int(q)] = (x < N) ? (float)gwl[(x < N ? x : 0) + imin - i + N - 1] : 0.f;
+      });
+      static_for<2>([&](auto X) {
+        const double mref = (X == 0) ? mref0 : mref1;
+        const float scf = (float)sc[X];
+        bf16x8 Ah[2], Al[2];
+        const double* fv = fva[X];
+        double csum = 0., wsum = 0.;
+        static_for<16>([&](auto q) {
           csum += (double)wr[int(q)] * fv[int(q)];  // wr is zero for x >= N
           wsum += (double)wr[int(q)];
         });
@@ -285,7 +301,8 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         wsum += __shfl_xor(wsum, 16);
         wsum += __shfl_xor(wsum, 32);
         const double cm = csum / wsum;
-        if (g1 == 0) cmr[128 * X + u] = colok ? (float)(cm - mref) : 0.f;
+        // the four lanes of a column hold the same value: no branch
+        cmr[128 * X + u] = colok ? (float)(cm - mref) : 0.f;
         static_for<2>([&](auto s) {
           float v[8];
           static_for<8>([&](auto e) {
@@ -307,11 +324,8 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           const int k = 16 * kt + n1;
           f32x4 tre = z4, tim = z4;
           static_for<2>([&](auto s) {
-            const int off = k * C::RS1 + 32 * s + 8 * g1;
-            const bf16x8 brh = ld8(D1 + 0 * C::kD1Plane + off), brl = ld8(D1 + 1 * C::kD1Plane + off);
-            const bf16x8 bih = ld8(D1 + 2 * C::kD1Plane + off), bil = ld8(D1 + 3 * C::kD1Plane + off);
-            tre = mfma3(Ah[s], Al[s], brh, brl, tre);
-            tim = mfma3(Ah[s], Al[s], bih, bil, tim);
+            tre = mfma3(Ah[s], Al[s], d1f[kt][s][0], d1f[kt][s][1], tre);
+            tim = mfma3(Ah[s], Al[s], d1f[kt][s][2], d1f[kt][s][3], tim);
           });
           // lane holds T(u = 16w + 4g + t, k = 16kt + n): parity t & 1,
           // half column uh = 8w + 2g + (t >> 1)
@@ -673,16 +687,15 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
               static_for<2>([&](auto pm) {
                 static_for<4>([&](auto t) {
                   const unsigned long long m = __ballot(fabsf(cv[NT][q][IT][pm][int(t)]) >= thr);
-                  if (m) {
-                    nge += __popcll(m);
-                    const int ln = __builtin_ctzll(m);
-                    const int ii = 16 * IT + (ln & 15);
-                    const int sx = (pm == 0) ? ii + Nh : ii;
-                    const int jj = 32 * NT + 8 * (ln >> 4) + 2 * t + q;
-                    const int sy = (jj < Nh) ? jj + Nh : jj - Nh;
-                    const unsigned L = (unsigned)(sx * N + sy);
-                    loc = (L < loc) ? L : loc;
-                  }
+                  // branch-free: a zero mask leaves nge and loc as they are
+                  nge += __popcll(m);
+                  const int ln = __builtin_ctzll(m | (1ull << 63));
+                  const int ii = 16 * IT + (ln & 15);
+                  const int sx = (pm == 0) ? ii + Nh : ii;
+                  const int jj = 32 * NT + 8 * (ln >> 4) + 2 * t + q;
+                  const int sy = (jj < Nh) ? jj + Nh : jj - Nh;
+                  const unsigned L = m ? (unsigned)(sx * N + sy) : 0xffffffffu;
+                  loc = (L < loc) ? L : loc;
                 });
               });
             });
