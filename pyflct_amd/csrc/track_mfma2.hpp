@@ -80,14 +80,41 @@ struct CF4 {
   f32x4 re, im;
 };
 
-// two floats -> packed BF16 hi pair and lo pair (hi = RNE(x), lo = RNE(x - hi))
-__device__ __forceinline__ void split2(float a, float b, unsigned& hi, unsigned& lo) {
+// two floats -> packed BF16 hi pair and lo pair (hi = RNE(x), lo = RNE(x - hi)).
+// The residuals come from one dot2 each: x + (-1) hi_x + 0 hi_y, whose exact
+// value x - hi_x is representable in FP32 (hi_x is x rounded to 8 bits), so
+// any rounding of the dot product returns it unchanged; 4 VALU instructions
+// per pair instead of unpacking hi (2) and subtracting (2).  The (-1, 0)
+// and (0, -1) operands are passed in scalar registers: as constants the
+// compiler encodes (-1, 0) as the inline constant -1.0, which v_dot2c_f32_bf16
+// on gfx950 reads as the FP32 bit pattern, i.e. (0, -1) (measured: wrong lo
+// parts for 32763 of 32768 random pairs; the register form matches the
+// unpack-and-subtract split on all of them)
+__device__ __forceinline__ void split2k(float a, float b, unsigned& hi, unsigned& lo, unsigned k0,
+                                        unsigned k1) {
   const bf16x2 h = __builtin_convertvector((f32x2v){a, b}, bf16x2);
-  const unsigned hu = __builtin_bit_cast(unsigned, h);
-  const float ah = __uint_as_float(hu << 16), bh = __uint_as_float(hu & 0xffff0000u);
-  const bf16x2 l = __builtin_convertvector((f32x2v){a - ah, b - bh}, bf16x2);
-  hi = hu;
+  const bf16x2 m0 = __builtin_bit_cast(bf16x2, k0), m1 = __builtin_bit_cast(bf16x2, k1);
+  const float ra = __builtin_amdgcn_fdot2_f32_bf16(h, m0, a, false);
+  const float rb = __builtin_amdgcn_fdot2_f32_bf16(h, m1, b, false);
+  const bf16x2 l = __builtin_convertvector((f32x2v){ra, rb}, bf16x2);
+  hi = __builtin_bit_cast(unsigned, h);
   lo = __builtin_bit_cast(unsigned, l);
+}
+// the two dot2 operands, opaque to the compiler (kept in scalar registers)
+struct SplitK {
+  unsigned k0, k1;
+};
+__device__ __forceinline__ SplitK split_consts() {
+  SplitK k{0x0000bf80u, 0xbf800000u};  // bf16 pairs (-1, 0), (0, -1)
+  asm volatile("" : "+s"(k.k0), "+s"(k.k1));
+  return k;
+}
+__device__ __forceinline__ void split2(float a, float b, unsigned& hi, unsigned& lo) {
+  const SplitK k = split_consts();
+  split2k(a, b, hi, lo, k.k0, k.k1);
+}
+__device__ __forceinline__ void split2(float a, float b, unsigned& hi, unsigned& lo, SplitK k) {
+  split2k(a, b, hi, lo, k.k0, k.k1);
 }
 
 __device__ __forceinline__ bf16x8 ld8(const __bf16* p) {

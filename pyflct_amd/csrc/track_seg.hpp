@@ -254,6 +254,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       // per-segment copies of the lane indices: keeps the table loads below
       // from being hoisted out of the segment loop
       const int n1 = fresh(n0), g1 = fresh(g0);
+      const SplitK sk = split_consts();
       const int u = 16 * ct + n1;
       const bool colok = u < ucols;
       const bool ones = (u == 127);
@@ -313,7 +314,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           u32x4 hv, lv;
           static_for<4>([&](auto d) {
             unsigned hh, ll;
-            split2(v[2 * d], v[2 * d + 1], hh, ll);
+            split2(v[2 * d], v[2 * d + 1], hh, ll, sk);
             hv[int(d)] = hh;
             lv[int(d)] = ll;
           });
@@ -333,8 +334,8 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           static_for<2>([&](auto par) {
             __bf16* pl = TT + (X * 2 + par) * TP + k * RK + uh;
             unsigned h0, l0, h1, l1;
-            split2(tre[int(par)], tre[int(par) + 2], h0, l0);
-            split2(tim[int(par)], tim[int(par) + 2], h1, l1);
+            split2(tre[int(par)], tre[int(par) + 2], h0, l0, sk);
+            split2(tim[int(par)], tim[int(par) + 2], h1, l1, sk);
             *reinterpret_cast<unsigned*>(pl + 0 * RW) = h0;
             *reinterpret_cast<unsigned*>(pl + 1 * RW) = l0;
             *reinterpret_cast<unsigned*>(pl + 2 * RW) = h1;
@@ -370,6 +371,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       // per-pixel copies of lane indices: keeps index math out of registers
       // across the pixel loop
       const int n = fresh(n0), g = fresh(g0);
+      const SplitK sk = split_consts();
       // lane bases: table row n, T row k = n, 8 contraction values from 8g;
       // every operand below is an immediate offset from one of them
       const __bf16* TBl = TB + (n * RS + 8 * g);
@@ -418,7 +420,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           u32x4 hv, lv;
           static_for<4>([&](auto d) {
             unsigned hh, ll;
-            split2(v[2 * d], v[2 * d + 1], hh, ll);
+            split2(v[2 * d], v[2 * d + 1], hh, ll, sk);
             hv[int(d)] = hh;
             lv[int(d)] = ll;
           });
@@ -596,8 +598,8 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
                 }
               });
               unsigned h0, l0, h1, l1;
-              split2(vr[0], vr[1], h0, l0);
-              split2(vi[0], vi[1], h1, l1);
+              split2(vr[0], vr[1], h0, l0, sk);
+              split2(vi[0], vi[1], h1, l1, sk);
               rh[2 * LT + th] = h0;
               rl[2 * LT + th] = l0;
               ih[2 * LT + th] = h1;
@@ -617,7 +619,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
             ui = mfma3(qih, qil, brh, brl, ui);
             static_for<4>([&](auto t) {
               unsigned hh, ll;
-              split2(ur[int(t)], ui[int(t)], hh, ll);
+              split2(ur[int(t)], ui[int(t)], hh, ll, sk);
               Uh[kt][NT][q][int(t)] = hh;
               Ul[kt][NT][q][int(t)] = ll;
             });
