@@ -422,7 +422,9 @@ def uneven_batch(args):
             paths = {}
             for nb, (n, sigma, d) in sorted(by_box.items()):
                 ctx.autotune(d["f1"], d["f2"], n, n, ctx.make_opts(1.0, 1.0, sigma, precision=form))
-                paths[nb] = ctx.last_path()
+                sl, sp = ctx.last_shape()
+                paths[nb] = ctx.last_path() + (f" slots {sl}" if sl else "") + (f" segment {sp}" if sp else "")
+                log(f"bench: {form} box {nb} ({n}^2): {paths[nb]}")
             tune_s = time.perf_counter() - t0
             desc = [(d["f1"], d["f2"], d[f"{form}_vx"], d[f"{form}_vy"], d[f"{form}_vm"], n, n,
                      ctx.make_opts(1.0, 1.0, sigma, precision=form)) for n, sigma, d in pairs]

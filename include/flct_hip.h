@@ -175,10 +175,19 @@ int flct_hip_last_kernel_ms(flct_hip_ctx *ctx, double *main_ms, double *refine_m
  * counterpart in the reference. */
 const char *flct_hip_last_path(flct_hip_ctx *ctx);
 
+/* Runtime shape of the last call's main tracking kernel: *slots, the
+ * pixels tracked together per block by the FFT kernels (fp64-fft, fp32-fft,
+ * fp64-rowseg), and *segpx, the pixels per row segment of the row-segment
+ * kernels (bf16-segment, fp64-rowseg); 0 stands for the compiled shape (its
+ * slot count, 64 pixels per segment).  No counterpart in the reference. */
+int flct_hip_last_shape(flct_hip_ctx *ctx, int *slots, int *segpx);
+
 /* Autotuner: runs every kernel path that meets the precision form of opts
- * on the given device inputs, times the whole call with HIP events, and
- * makes the fastest the context's path for that form and box size (until
- * the context is destroyed or tuned again).  Untuned, the split-BF16 form
+ * on the given device inputs, times the whole call with HIP events, keeps
+ * the fastest, then times that path's runtime shapes (half, a quarter and
+ * one of its compiled slots per block; segments of 32 and 16 pixels) and
+ * makes the fastest configuration the context's for that form and box size
+ * (until the context is destroyed or tuned again).  Untuned, the split-BF16 form
  * runs the FP32 kernels for boxes up to 20 and above 64 and the row-segment
  * kernel from 24 to 64 (measured fastest there); its per-pixel MFMA kernels
  * are eligible up to box 72, above which their error can exceed the form's
@@ -193,6 +202,13 @@ int flct_hip_autotune(flct_hip_ctx *ctx, const double *d_f1, const double *d_f2,
  * otherwise); NULL or "" restores the untuned default.  No counterpart in
  * the reference. */
 int flct_hip_set_path(flct_hip_ctx *ctx, int precision, int nbox, const char *path);
+
+/* flct_hip_set_path with a runtime shape: slots 0 (compiled count) or 1 up
+ * to the path's compiled slots per block, segpx 0 (64) or 8..64 on the
+ * row-segment paths; FLCT_HIP_E_ARG for a shape the path does not have.
+ * No counterpart in the reference. */
+int flct_hip_set_config(flct_hip_ctx *ctx, int precision, int nbox, const char *path, int slots,
+                        int segpx);
 
 /*
  * flct_hip_warp replaces warp_frac2d (warp.c:883-1011): out(i,j) is the

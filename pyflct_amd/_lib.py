@@ -50,6 +50,8 @@ EXPORTS = (
     "flct_hip_last_path",
     "flct_hip_autotune",
     "flct_hip_set_path",
+    "flct_hip_set_config",
+    "flct_hip_last_shape",
     "flct_hip_warp",
     "flct_hip_warp_device",
     "flct_hip_set_warp_method",
@@ -137,6 +139,8 @@ def lib():
         L.flct_hip_last_path.restype = ctypes.c_char_p
         L.flct_hip_autotune.argtypes = [vp, vp, vp, i, i, ctypes.POINTER(Opts)]
         L.flct_hip_set_path.argtypes = [vp, i, i, ctypes.c_char_p]
+        L.flct_hip_set_config.argtypes = [vp, i, i, ctypes.c_char_p, i, i]
+        L.flct_hip_last_shape.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i)]
         L.flct_hip_warp.argtypes = [vp, _dp, _dp, _dp, _dp, i, i, i, i]
         L.flct_hip_warp_device.argtypes = [vp, vp, vp, vp, vp, i, i, i]
         L.flct_hip_set_warp_method.argtypes = [vp, i]

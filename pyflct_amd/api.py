@@ -224,7 +224,8 @@ class Context:
 
     def autotune(self, d_f1, d_f2, nx, ny, opts):
         """Time every kernel path that meets opts' precision form on these
-        device inputs and keep the fastest for that form and box size."""
+        device inputs, then the winner's runtime shapes (slots per block,
+        segment width), and keep the fastest for that form and box size."""
         check(_lib.lib().flct_hip_autotune(self._h, d_f1, d_f2, int(nx), int(ny),
                                            ctypes.byref(opts)), "flct_hip_autotune")
         return self.last_path()
@@ -236,6 +237,24 @@ class Context:
         check(_lib.lib().flct_hip_set_path(self._h, prec, int(nbox),
                                            None if path is None else path.encode()),
               "flct_hip_set_path")
+
+    def set_config(self, precision, nbox, path, slots=0, segpx=0):
+        """Pin a kernel path and its runtime shape (flct_hip_set_config):
+        slots per block (0: compiled count) and pixels per row segment
+        (0: 64); ``path=None`` restores the default."""
+        prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        check(_lib.lib().flct_hip_set_config(self._h, prec, int(nbox),
+                                             None if path is None else path.encode(),
+                                             int(slots), int(segpx)),
+              "flct_hip_set_config")
+
+    def last_shape(self):
+        """(slots per block, pixels per segment) of the last call's main
+        kernel; 0 stands for the compiled shape."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(_lib.lib().flct_hip_last_shape(self._h, ctypes.byref(a), ctypes.byref(b)),
+              "flct_hip_last_shape")
+        return a.value, b.value
 
     def last_path(self):
         """Kernel path of the last track call ("fp64-fft", "fp32-fft",

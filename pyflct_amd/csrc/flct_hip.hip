@@ -43,6 +43,16 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// A kernel configuration: the path and its runtime shape.  slots: pixels
+// tracked together per block by k_track / k_track_rs (1 up to the compiled
+// count; 0: the compiled count).  segpx: pixels per row segment of
+// k_track_seg / k_track_rs (8..64; 0: 64).
+struct Tuning {
+  int path = -1;
+  int slots = 0;
+  int segpx = 0;
+};
+
 }  // namespace
 
 struct flct_hip_ctx {
@@ -67,12 +77,13 @@ struct flct_hip_ctx {
   int seg_n = 0;          // box size of the split tables in segtab
   double seg_sigma = 0.;  // window width of the split tables
   int bf16_kernel = 0;    // FLCT_HIP_BF16_KERNEL: 0 automatic, 1 per-pixel kernels only
-  // kernel path by (precision, box size, bias), chosen by flct_hip_autotune;
-  // -h always runs the FP64 kernels and is never stored
-  std::map<std::tuple<int, int, int>, int> tuned;
+  // kernel configuration by (precision, box size, bias), chosen by
+  // flct_hip_autotune; -h always runs the FP64 kernels and is never stored
+  std::map<std::tuple<int, int, int>, Tuning> tuned;
   int tune_gen = 0;       // bumped by every tuning (part of the batch graph key)
-  int force_path = -1;    // path under test while tuning
+  Tuning force;           // configuration under test while tuning (path -1: none)
   int last_path = 0;
+  int last_slots = 0, last_segpx = 0;  // runtime shape of the last call's main kernel
   long long last_tracked = 0;
   long long refined = 0;  // pixels re-evaluated in FP64 by the last split-BF16 or FP32 call
   double last_kernel_ms = 0.;
@@ -378,13 +389,15 @@ struct Plan {
   int path = PATH_FP64;   // kernel path
   int ixc = 0, iyc = 0;   // skip-fill lattice size
   long long big_grid = 0; // blocks of k_track_big (boxes above 96)
-  int rs_nband = 0;       // k_track_rs: column bands of at most kRsSeg pixels
+  int rs_nband = 0;       // k_track_rs: column bands of at most segpx pixels
   long long rs_grid = 0;  // k_track_rs: blocks
+  int slots = 0;          // k_track / k_track_rs pixels per block (0: compiled count)
+  int segpx = 0;          // k_track_seg / k_track_rs pixels per segment (0: 64)
   std::vector<double> gw, filt;
 };
 
 int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Plan* pl,
-              int force_path = -1) {
+              const Tuning* force = nullptr) {
   if (!o || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
   if (o->deltat == 0.) return FLCT_HIP_E_ARG;
   int rc = resolve(o, nx, ny, &pl->r);
@@ -431,12 +444,16 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Pla
   }
   pl->relative = (o->thresh > 0.) && (o->thresh < 1.) && (!o->thresh_abs);
   pl->sparse = (o->thresh > 0.) || pl->r.skipon;
-  if (force_path >= 0) {
-    pl->path = force_path;
+  if (force && force->path >= 0) {
+    pl->path = force->path;
+    pl->slots = force->slots;
+    pl->segpx = force->segpx;
   } else {
     const auto it = c->tuned.find(std::make_tuple(o->precision, N, o->bias ? 1 : 0));
     if (it != c->tuned.end()) {
-      pl->path = it->second;
+      pl->path = it->second.path;
+      pl->slots = it->second.slots;
+      pl->segpx = it->second.segpx;
     } else {
       pl->path = default_path(o->precision, N, nx, ny, o->bias, c->bf16_kernel);
       // FP64 without a threshold: the row-shared kernel where it was faster
@@ -448,8 +465,9 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Pla
   if (pl->path == PATH_BF16_SEG && !seg_supported(N, nx, ny, o->bias)) pl->path = PATH_BF16_PIXEL;
   if (pl->path == PATH_FP64_ROWSEG) {
     if (rowseg_ok(N, o->bias, o->hires, pl->r.skipon)) {
-      pl->rs_nband = (ny + kRsSeg - 1) / kRsSeg;
-      pl->rs_grid = rs_grid(N / 4, c->num_cu, (long long)nx * pl->rs_nband);
+      const int w = (pl->segpx >= 8 && pl->segpx < kRsSeg) ? pl->segpx : kRsSeg;
+      pl->rs_nband = (ny + w - 1) / w;
+      pl->rs_grid = rs_grid(N / 4, c->num_cu, (long long)nx * pl->rs_nband, pl->slots);
     } else {
       pl->path = PATH_FP64;
     }
@@ -585,6 +603,7 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
   ta.vy = vy;
   ta.vm = vm;
   ta.absmax = bf16 ? absmax : nullptr;
+  ta.slots = pl.slots;
   // the split-BF16 and FP32 kernels list the pixels their surface cannot
   // settle; those are tracked again by the FP64 kernels
   const bool refine = bf16 || pl.path == PATH_FP32 || pl.path == PATH_FP64_ROWSEG;
@@ -639,13 +658,14 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
         return rc;
       if (nedge != 0) {
         TrackArgs te = ta;
+        te.slots = 0;
         te.list = sc.elist;
         te.nwork = (nedge > 0) ? nedge : npix;
         te.nwork_dev = (nedge > 0) ? nullptr : counts + 2;
         if ((rc = dispatch_track_bf16(N, lc, te))) return rc;
       }
       if ((rc = launch_track_seg(lc, ta, tb.segtab, o->thresh, pl.relative ? 1 : 0,
-                                 pl.r.skipon ? o->skip : 0, xoffset, yoffset)))
+                                 pl.r.skipon ? o->skip : 0, xoffset, yoffset, pl.segpx)))
         return rc;
     } else if ((rc = dispatch_track_bf16(N, lc, ta))) {
       return rc;
@@ -660,6 +680,7 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
     if (ev) HIPCHK(hipEventRecord(ev[4], s));
     // FP64 re-evaluation of the flagged pixels (count read on the device)
     TrackArgs tr = ta;
+    tr.slots = 0;  // the re-evaluation runs the compiled shape
     tr.list = sc.flags;
     tr.nwork = npix;
     tr.nwork_dev = counts + 1;
@@ -900,7 +921,7 @@ extern "C" int flct_hip_track_device(flct_hip_ctx* c, const double* f1, const do
     }
   }
   Plan pl;
-  int rc = make_plan(nx, ny, o, c, &pl, c->force_path);
+  int rc = make_plan(nx, ny, o, c, &pl, &c->force);
   if (rc) return rc;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
@@ -952,6 +973,8 @@ static int end_call(flct_hip_ctx* c, const Plan& pl, const unsigned int* counts)
   const bool refine = (pl.path == PATH_FP32 || pl.path == PATH_BF16_SEG ||
                        pl.path == PATH_BF16_PIXEL || pl.path == PATH_FP64_ROWSEG);
   c->last_path = pl.path;
+  c->last_slots = pl.slots;
+  c->last_segpx = pl.segpx;
   c->last_tracked = pl.sparse ? (long long)hc[0] : (long long)nx * ny;
   c->refined = refine ? (long long)hc[1] : 0;
   c->last_kernel_ms = ev_ms(c->ev[1], c->ev[2]);
@@ -985,7 +1008,7 @@ extern "C" int flct_hip_track_device_noalloc(flct_hip_ctx* c, const double* f1, 
   }
   if (!c->noalloc) return FLCT_HIP_E_HIP;
   Plan pl;
-  int rc = make_plan(nx, ny, o, c, &pl, c->force_path);
+  int rc = make_plan(nx, ny, o, c, &pl, &c->force);
   if (rc) return rc;
   const int N = pl.N;
   const long long npix = (long long)nx * ny;
@@ -1114,7 +1137,36 @@ extern "C" const char* flct_hip_last_path(flct_hip_ctx* c) {
   return c ? path_name(c->last_path) : "none";
 }
 
-extern "C" int flct_hip_set_path(flct_hip_ctx* c, int precision, int nbox, const char* path) {
+extern "C" int flct_hip_last_shape(flct_hip_ctx* c, int* slots, int* segpx) {
+  if (!c) return FLCT_HIP_E_ARG;
+  if (slots) *slots = c->last_slots;
+  if (segpx) *segpx = c->last_segpx;
+  return FLCT_HIP_OK;
+}
+
+// compiled slots per block of a path's main kernel at box N (0: the path
+// has no slot axis)
+static int path_slots(int path, int N) {
+  const int M = N / 4;
+  if (N % 4 != 0 || M < 1 || M > kTrackMaxM) return 0;
+  if (path == PATH_FP64) return track_slots<double>(M);
+  if (path == PATH_FP32) return track_slots<float>(M);
+  if (path == PATH_FP64_ROWSEG) return rs_slots(M);
+  return 0;
+}
+
+static bool path_has_segpx(int path) { return path == PATH_BF16_SEG || path == PATH_FP64_ROWSEG; }
+
+// a runtime shape the path accepts: slots 0 or 1..its compiled count,
+// segpx 0 or 8..64 on the row-segment paths
+static bool shape_ok(int path, int N, int slots, int segpx) {
+  if (slots != 0 && (slots < 1 || slots > path_slots(path, N))) return false;
+  if (segpx != 0 && (!path_has_segpx(path) || segpx < 8 || segpx > 64)) return false;
+  return true;
+}
+
+extern "C" int flct_hip_set_config(flct_hip_ctx* c, int precision, int nbox, const char* path,
+                                   int slots, int segpx) {
   if (!c || nbox < 4 || nbox % 4 != 0 || nbox > kBigMaxN) return FLCT_HIP_E_ARG;
   if (precision < FLCT_HIP_FP64 || precision > FLCT_HIP_SPLIT_BF16) return FLCT_HIP_E_ARG;
   // one choice for calls with and without -b (a -b call the row-segment
@@ -1134,13 +1186,25 @@ extern "C" int flct_hip_set_path(flct_hip_ctx* c, int precision, int nbox, const
   // also needs a box of 8..64 (calls it cannot serve run the per-pixel one)
   bool ok = false;
   for (int q : path_candidates(precision, nbox, 1 << 20, 1 << 20, 0, 0)) ok = ok || (q == p);
-  if (!ok) return FLCT_HIP_E_ARG;
-  c->tuned[key] = p;
-  c->tuned[keyb] = p;
+  if (!ok || !shape_ok(p, nbox, slots, segpx)) return FLCT_HIP_E_ARG;
+  Tuning t;
+  t.path = p;
+  t.slots = slots;
+  t.segpx = segpx;
+  c->tuned[key] = t;
+  c->tuned[keyb] = t;
   ++c->tune_gen;
   return FLCT_HIP_OK;
 }
 
+extern "C" int flct_hip_set_path(flct_hip_ctx* c, int precision, int nbox, const char* path) {
+  return flct_hip_set_config(c, precision, nbox, path, 0, 0);
+}
+
+// Autotuner.  Stage 1 times every candidate path at its compiled shape;
+// stage 2 times the winner's runtime shapes (slots per block, segment
+// width) around the compiled one.  Each configuration runs three times on
+// the caller's inputs and the last two are summed (the first warms up).
 extern "C" int flct_hip_autotune(flct_hip_ctx* c, const double* f1, const double* f2, int nx,
                                  int ny, const flct_hip_opts* o) {
   if (!c || !f1 || !f2 || !o || nx <= 0 || ny <= 0) return FLCT_HIP_E_ARG;
@@ -1157,21 +1221,55 @@ extern "C" int flct_hip_autotune(flct_hip_ctx* c, const double* f1, const double
   const size_t bytes = (size_t)nx * ny * 8;
   if ((rc = ensure(c->vx, bytes)) || (rc = ensure(c->vy, bytes)) || (rc = ensure(c->vm, bytes)))
     return rc;
-  int best = -1;
+  auto time_cfg = [&](const Tuning& t, double* ms) {
+    c->force = t;
+    double sum = 0.;
+    int r = FLCT_HIP_OK;
+    for (int rep = 0; rep < 3 && !r; ++rep) {
+      r = flct_hip_track_device(c, f1, f2, nx, ny, o, (double*)c->vx.p, (double*)c->vy.p,
+                                (double*)c->vm.p);
+      if (rep > 0) sum += c->last_total_ms;
+    }
+    c->force = Tuning();
+    *ms = sum;
+    return r;
+  };
+  Tuning best;
   double best_ms = 0.;
   for (int p : path_candidates(o->precision, pl.N, nx, ny, o->bias, o->hires, pl.r.skipon)) {
-    c->force_path = p;
-    double t = 0.;
-    for (int rep = 0; rep < 3 && !rc; ++rep) {  // first run warms up
-      rc = flct_hip_track_device(c, f1, f2, nx, ny, o, (double*)c->vx.p, (double*)c->vy.p,
-                                 (double*)c->vm.p);
-      if (rep > 0) t += c->last_total_ms;
+    Tuning t;
+    t.path = p;
+    double ms = 0.;
+    if ((rc = time_cfg(t, &ms))) return rc;
+    if (best.path < 0 || ms < best_ms) {
+      best = t;
+      best_ms = ms;
     }
-    c->force_path = -1;
-    if (rc) return rc;
-    if (best < 0 || t < best_ms) {
-      best = p;
-      best_ms = t;
+  }
+  // stage 2: the winner's shapes (fewer slots per block give more blocks
+  // on small images; narrower segments more segments per row)
+  std::vector<Tuning> shapes;
+  const int P = path_slots(best.path, pl.N);
+  for (int sl : {P / 2, P / 4, 1})
+    if (sl >= 1 && sl < P) {
+      Tuning t = best;
+      t.slots = sl;
+      bool dup = false;
+      for (const Tuning& u : shapes) dup = dup || (u.slots == sl && u.segpx == 0);
+      if (!dup) shapes.push_back(t);
+    }
+  if (path_has_segpx(best.path))
+    for (int w : {32, 16}) {
+      Tuning t = best;
+      t.segpx = w;
+      shapes.push_back(t);
+    }
+  for (const Tuning& t : shapes) {
+    double ms = 0.;
+    if ((rc = time_cfg(t, &ms))) return rc;
+    if (ms < best_ms) {
+      best = t;
+      best_ms = ms;
     }
   }
   // -h forces the FP64 kernels whatever is stored: nothing to keep
@@ -1179,7 +1277,9 @@ extern "C" int flct_hip_autotune(flct_hip_ctx* c, const double* f1, const double
     c->tuned[std::make_tuple(o->precision, pl.N, o->bias ? 1 : 0)] = best;
     c->tune_gen++;
   }
-  c->last_path = best;
+  c->last_path = best.path;
+  c->last_slots = best.slots;
+  c->last_segpx = best.segpx;
   return FLCT_HIP_OK;
 }
 

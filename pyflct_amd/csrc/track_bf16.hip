@@ -191,7 +191,7 @@ int launch_edge_list(const LaunchCtx& c, const int* list, const unsigned int* nl
 }
 
 int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, double thresh,
-                     int thresh_rel, int skip, int xoffset, int yoffset) {
+                     int thresh_rel, int skip, int xoffset, int yoffset, int segpx) {
   using C = SegCfg;
   const int N = 2 * a.nsize;
   SegArgs s;
@@ -204,7 +204,8 @@ int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, do
   s.yoffset = yoffset;
   s.jlo = a.nsize - 1;
   s.width = a.ny - N + 1;
-  s.nband = (s.width + C::P - 1) / C::P;
+  const int w = (segpx >= 8 && segpx < C::P) ? segpx : C::P;
+  s.nband = (s.width + w - 1) / w;
   const long long nseg = (long long)s.nband * a.nx;
   if (nseg > 0x7fffffffLL) return FLCT_HIP_E_UNSUPPORTED;
   s.nseg = (int)nseg;

@@ -55,7 +55,13 @@ struct TrackArgs {
   double* vx;
   double* vy;
   double* vm;
+  int slots;  // pixels tracked together per block, 1..the compiled count (0: compiled count)
 };
+
+// slots per block of a launch: the tuned count when it is 1..P, else P
+__device__ __host__ __forceinline__ int run_slots(int slots, int P) {
+  return (slots > 0 && slots < P) ? slots : P;
+}
 
 // The FP32 form lists the pixels its surface cannot settle for re-evaluation
 // in FP64, like the split-BF16 form: another cell within kTieTol32 of the
@@ -215,7 +221,8 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads), (TrackCfg<M, T>::k
   const int tid = threadIdx.x;
   const int slot = tid / Cfg::kSlotThreads;
   const int stid = tid - slot * Cfg::kSlotThreads;
-  const bool in_slot = slot < Cfg::P;
+  const int SP = run_slots(a.slots, Cfg::P);  // slots of this launch
+  const bool in_slot = slot < SP;
   const int lane = stid & 3, qd = stid >> 2;
   double* gw = reinterpret_cast<double*>(lds);
   for (int t = tid; t < 2 * N - 1; t += blockDim.x) gw[t] = a.gw[t];
@@ -251,12 +258,12 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads), (TrackCfg<M, T>::k
     const long long nbx = (gridDim.x - x + nband - 1) / nband;
     r0 = (int)((long long)nx * x / nband);
     rows_b = (int)((long long)nx * (x + 1) / nband) - r0;
-    first = (long long)(blockIdx.x / nband) * Cfg::P + slot;
-    stride = nbx * Cfg::P;
+    first = (long long)(blockIdx.x / nband) * SP + slot;
+    stride = nbx * SP;
     count = (long long)rows_b * ny;
   } else {
-    first = (long long)blockIdx.x * Cfg::P + slot;
-    stride = (long long)gridDim.x * Cfg::P;
+    first = (long long)blockIdx.x * SP + slot;
+    stride = (long long)gridDim.x * SP;
     count = a.nwork_dev ? (long long)*a.nwork_dev : a.nwork;
   }
   const long long iters = (count + stride - 1) / stride;

@@ -19,24 +19,40 @@ struct LaunchCtx {
   size_t lds_per_block;
 };
 
+// Blocks hold a.slots pixels (1..P, 0: the compiled count P); LDS and
+// threads per block follow the slot count, and a CU takes as many blocks as
+// its 160 KB of LDS and 2048 threads hold (at most 8).
 template <int M, class T, bool HIRES = false>
 int launch_track(const LaunchCtx& c, const TrackArgs& a) {
   using Cfg = TrackCfg<M, T>;
-  const size_t lds = Cfg::kLdsBytes;
+  const int sp = run_slots(a.slots, Cfg::P);
+  const size_t lds = (size_t)sp * Cfg::kSlotBytes + 2 * Cfg::N * 8;
+  const int threads = (sp * Cfg::kSlotThreads + 63) / 64 * 64;
   if (lds > c.lds_per_block) return FLCT_HIP_E_UNSUPPORTED;
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void*)k_track<M, T, HIRES>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   long long per_cu = (long long)(160 * 1024) / (long long)lds;
+  if (per_cu > 2048 / threads) per_cu = 2048 / threads;
   if (per_cu < 1) per_cu = 1;
   if (per_cu > 8) per_cu = 8;
   long long grid = (long long)c.num_cu * per_cu;
-  const long long need = (a.nwork + Cfg::P - 1) / Cfg::P;
+  const long long need = (a.nwork + sp - 1) / sp;
   if (grid > need) grid = need;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((k_track<M, T, HIRES>), dim3((unsigned)grid), dim3(Cfg::kThreads), lds, c.stream,
-                     a);
+  hipLaunchKernelGGL((k_track<M, T, HIRES>), dim3((unsigned)grid), dim3(threads), lds, c.stream, a);
   return hipGetLastError() == hipSuccess ? FLCT_HIP_OK : FLCT_HIP_E_HIP;
+}
+
+// compiled slots per block of k_track for box size 4*M (0: no such size)
+template <class T>
+int track_slots(int M) {
+  int p = 0;
+  static_for<kTrackMaxM>([&](auto I) {
+    constexpr int m = I + 1;
+    if (m == M) p = TrackCfg<m, T>::P;
+  });
+  return p;
 }
 
 // dispatchers, one per translation unit: box sizes M = 1..12 and 13..24
@@ -61,7 +77,8 @@ bool seg_supported(int N, int nx, int ny, int bias);
 void seg_tables(int N, double sigma, std::vector<unsigned short>& tab);
 int launch_edge_list(const LaunchCtx& c, const int* list, const unsigned int* nlist, int nx, int ny,
                      int nsize, int* out, unsigned int* count, long long* nedge);
+// segpx: pixels per row segment, 8..64 (0: 64)
 int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, double thresh,
-                     int thresh_rel, int skip, int xoffset, int yoffset);
+                     int thresh_rel, int skip, int xoffset, int yoffset, int segpx);
 
 }  // namespace flct

@@ -90,7 +90,8 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
   const int tid = threadIdx.x;
   const int slot = tid / Cfg::kSlotThreads;
   const int stid = tid - slot * Cfg::kSlotThreads;
-  const bool in_slot = slot < Cfg::P;
+  const int SP = run_slots(A.t.slots, Cfg::P);  // slots of this launch
+  const bool in_slot = slot < SP;
   const int lane = stid & 3, qd = stid >> 2;
   double* gw = reinterpret_cast<double*>(lds);                     // 2N - 1
   cplx<T>* wrh = reinterpret_cast<cplx<T>*>(gw + 2 * N);           // N: Wr^(k)
@@ -169,7 +170,7 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
     // u < U, and Wr^(k) from a column of ones (u = U)
     if (in_slot) {
       const int l = fresh(lane);
-      const int nq = Cfg::P * (Cfg::kSlotThreads / 4);
+      const int nq = SP * (Cfg::kSlotThreads / 4);
       const int kbl = M * brev2(l);
       const double* p1 = a.f1 + (long long)imin * ny + cb;
       const double* p2 = a.f2 + (long long)imin * ny + cb;
@@ -226,7 +227,7 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
     __syncthreads();
 
     // ------------- the segment's tracked pixels, one per slot at a time
-    for (int rp = 0; rp < cnt; rp += Cfg::P) {
+    for (int rp = 0; rp < cnt; rp += SP) {
       const bool active = in_slot && (rp + slot) < cnt;
       int j = 0, jmin = 0;
       double d1 = 0., d2 = 0., f1b = 0., f2b = 0.;
@@ -469,31 +470,51 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
   }
 }
 
+// LDS bytes and threads of a block of sp slots (1..P)
+template <int M>
+static void rs_block(int sp, size_t* lds, int* threads) {
+  using Cfg = RsCfg<M>;
+  *lds = (size_t)sp * Cfg::kSlotBytes + Cfg::kShared;
+  *threads = (sp * Cfg::kSlotThreads + 63) / 64 * 64;
+}
+
 template <int M>
 static int launch_rs(const LaunchCtx& c, const RsArgs& a, long long grid) {
-  using Cfg = RsCfg<M>;
-  const size_t lds = Cfg::kLdsBytes;
+  size_t lds = 0;
+  int threads = 0;
+  rs_block<M>(run_slots(a.t.slots, RsCfg<M>::P), &lds, &threads);
   if (lds > c.lds_per_block) return FLCT_HIP_E_UNSUPPORTED;
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void*)k_track_rs<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
-  hipLaunchKernelGGL(k_track_rs<M>, dim3((unsigned)grid), dim3(Cfg::kThreads), lds, c.stream, a);
+  hipLaunchKernelGGL(k_track_rs<M>, dim3((unsigned)grid), dim3(threads), lds, c.stream, a);
   return hipGetLastError() == hipSuccess ? FLCT_HIP_OK : FLCT_HIP_E_HIP;
 }
 
-long long rs_grid(int M, int num_cu, long long nseg) {
-  long long lds = 0;
+long long rs_grid(int M, int num_cu, long long nseg, int slots) {
+  size_t lds = 0;
+  int threads = 0;
   static_for<kTrackMaxM>([&](auto I) {
     constexpr int m = I + 1;
-    if (m == M) lds = RsCfg<m>::kLdsBytes;
+    if (m == M) rs_block<m>(run_slots(slots, RsCfg<m>::P), &lds, &threads);
   });
-  if (lds <= 0) return 0;
-  long long per_cu = (160LL * 1024) / lds;
+  if (lds == 0) return 0;
+  long long per_cu = (160LL * 1024) / (long long)lds;
+  if (per_cu > 2048 / threads) per_cu = 2048 / threads;
   if (per_cu < 1) per_cu = 1;
   if (per_cu > 4) per_cu = 4;
   long long g = (long long)num_cu * per_cu;
   if (g > nseg) g = nseg;
   return g < 1 ? 1 : g;
+}
+
+int rs_slots(int M) {
+  int p = 0;
+  static_for<kTrackMaxM>([&](auto I) {
+    constexpr int m = I + 1;
+    if (m == M) p = RsCfg<m>::P;
+  });
+  return p;
 }
 
 int dispatch_track_rs(int M, const LaunchCtx& c, const RsArgs& a, long long grid) {

@@ -19,8 +19,11 @@ struct RsArgs {
   double2* phi = nullptr;  // per block: N x ustride complex
 };
 
-// blocks of the persistent grid for box size 4*M (0: no such box)
-long long rs_grid(int M, int num_cu, long long nseg);
+// blocks of the persistent grid for box size 4*M with `slots` pixels per
+// block (0: the compiled count; 0 blocks: no such box)
+long long rs_grid(int M, int num_cu, long long nseg, int slots);
+// compiled slots per block of k_track_rs for box size 4*M
+int rs_slots(int M);
 int dispatch_track_rs(int M, const LaunchCtx& c, const RsArgs& a, long long grid);
 
 }  // namespace flct

@@ -100,7 +100,7 @@ struct SegCfg {
   static constexpr int kScratch = 128 * 8 + 64 * 8 + 9 * 8 + 8;  // zx, zp, nb, pad
   static constexpr int kGw = 128;  // window table gw[d + N - 1], 2N - 1 <= 127 doubles
   static constexpr int kLdsBytes =
-      kTabEl * 2 + 256 * 4 + kTEl * 2 + 68 * 4 + kWaves * kScratch + kGw * 8;
+      kTabEl * 2 + 256 * 8 + kTEl * 2 + 68 * 4 + kWaves * kScratch + kGw * 8;
   static_assert(kLdsBytes <= 160 * 1024, "k_track_seg needs more LDS than a CU has");
 };
 
@@ -153,8 +153,8 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
   const int n0 = lane & 15, g0 = lane >> 4;
 
   __bf16* TB = reinterpret_cast<__bf16*>(lds);
-  float* cmr = reinterpret_cast<float*>(TB + C::kTabEl);  // [X][u] cm(u) - mref
-  __bf16* TT = reinterpret_cast<__bf16*>(cmr + 256);
+  double* cmd = reinterpret_cast<double*>(TB + C::kTabEl);  // [X][u] cm(u)
+  __bf16* TT = reinterpret_cast<__bf16*>(cmd + 256);
   int* slist = reinterpret_cast<int*>(TT + C::kTEl);
   int* scount = slist + 64;
   double* gwl = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(slist + 68) +
@@ -246,8 +246,6 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
 
     const int cb = jb0 - (nsize - 1);  // image column of T column 0
     const int ucols = P + N - 1;       // T columns the boxes touch
-    const double mref0 = a.mean1[(long long)imin * ns + cb];
-    const double mref1 = a.mean2[(long long)imin * ns + cb];
 
     // ---------------- stage 1: column tile ct (T columns 16ct .. 16ct+15)
     for (int ct = wave; ct < 8; ct += C::kWaves) {
@@ -286,7 +284,6 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         wr[int(q)] = (x < N) ? (float)gwl[(x < N ? x : 0) + imin - i + N - 1] : 0.f;
       });
       static_for<2>([&](auto X) {
-        const double mref = (X == 0) ? mref0 : mref1;
         const float scf = (float)sc[X];
         bf16x8 Ah[2], Al[2];
         const double* fv = fva[X];
@@ -303,7 +300,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         wsum += __shfl_xor(wsum, 32);
         const double cm = csum / wsum;
         // the four lanes of a column hold the same value: no branch
-        cmr[128 * X + u] = colok ? (float)(cm - mref) : 0.f;
+        cmd[128 * X + u] = colok ? cm : 0.;
         static_for<2>([&](auto s) {
           float v[8];
           static_for<8>([&](auto e) {
@@ -364,10 +361,9 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       const int p = __builtin_amdgcn_readfirstlane(slist[qi]);
       const int j = jb0 + p;
       const int jmin = j - (nsize - 1);
-      // box-mean offsets from the segment's reference means (the ones
-      // column already carries the image scale)
-      const float dl[2] = {(float)(a.mean1[(long long)imin * ns + jmin] - mref0),
-                           (float)(a.mean2[(long long)imin * ns + jmin] - mref1)};
+      // box means (the ones column already carries the image scale)
+      const double fbar[2] = {a.mean1[(long long)imin * ns + jmin],
+                              a.mean2[(long long)imin * ns + jmin]};
       // per-pixel copies of lane indices: keeps index math out of registers
       // across the pixel loop
       const int n = fresh(n0), g = fresh(g0);
@@ -414,9 +410,11 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       bf16x8 cvh[2][2], cvl[2][2];  // [X][sset]
       static_for<2>([&](auto X) {
         static_for<2>([&](auto sset) {
-          const float* cr = cmr + 128 * X + p + sset + 16 * g;
+          // formed in FP64 and rounded once: the column means and the box
+          // mean can be large beside their difference
+          const double* cr = cmd + 128 * X + p + sset + 16 * g;
           float v[8];
-          static_for<8>([&](auto e) { v[int(e)] = cr[2 * e] - dl[X]; });
+          static_for<8>([&](auto e) { v[int(e)] = (float)(cr[2 * e] - fbar[X]); });
           u32x4 hv, lv;
           static_for<4>([&](auto d) {
             unsigned hh, ll;
