@@ -138,7 +138,10 @@ typedef struct flct_hip_pair {
  * equivalent of npairs invocations of the reference flct (flct.c:261-1154,
  * one pair per process), each with its own size and options.  All pairs'
  * kernels are captured into one HIP graph on the context's stream and
- * launched once; the graph is kept and relaunched while the next call
+ * launched once.  Inside the graph the pairs run on up to four concurrent
+ * branches (largest estimated work first, each pair to the least loaded
+ * branch), each branch with its own scratch; output arrays of different
+ * pairs must not overlap.  The graph is kept and relaunched while the next call
  * brings the same pairs (arrays, sizes, options).  Every pair is validated
  * first: if any is invalid, nothing runs, each pair's status is set and the
  * first error is returned.  flct_hip_last_stats reports the sum of the

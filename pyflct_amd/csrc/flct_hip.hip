@@ -69,8 +69,13 @@ struct flct_hip_ctx {
     DevBuf buf;  // twiddles (M/2) | chirp (n) | chirp transform (M), as double2
   };
   std::vector<DftTab> dft_tabs;
-  // batch: own scratch, table arena and counters, and the cached graph
-  DevBuf bscr[8], barena, bcount;
+  // batch: the pairs run on up to kBranches concurrent branches of one
+  // graph, each with its own stream (during capture) and scratch; the
+  // table arena and the per-pair counters are shared; the cached graph
+  static constexpr int kBranches = 4;
+  DevBuf bscr[kBranches][8], barena, bcount;
+  hipStream_t bstream[kBranches] = {};
+  hipEvent_t bev[kBranches + 1] = {};  // fork, then one join per branch
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<unsigned char> gkey;  // pairs the cached graph was captured for
@@ -188,8 +193,13 @@ extern "C" int flct_hip_destroy(flct_hip_ctx* c) {
                     &c->nuTab, &c->bigp};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
-  for (DevBuf& b : c->bscr)
-    if (b.p) (void)hipFree(b.p);
+  for (auto& set : c->bscr)
+    for (DevBuf& b : set)
+      if (b.p) (void)hipFree(b.p);
+  for (auto& st : c->bstream)
+    if (st) (void)hipStreamDestroy(st);
+  for (auto& e : c->bev)
+    if (e) (void)hipEventDestroy(e);
   for (auto& t : c->dft_tabs)
     if (t.buf.p) (void)hipFree(t.buf.p);
   if (c->barena.p) (void)hipFree(c->barena.p);
@@ -1343,17 +1353,45 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
       }
     }
   }
-  // scratch for the largest pair, counters per pair
-  size_t need[kNumScratch];
-  for (int i = 0; i < kNumScratch; ++i) need[i] = 256;
+  // Branches: pairs in decreasing order of estimated work (pixels times the
+  // FFT-form flops per pixel of their box), each to the branch with the
+  // least work so far (longest processing time first).  The branches run
+  // concurrently; inside a branch the pairs run in that order and share the
+  // branch's scratch.
+  const int nbr = std::min(npairs, flct_hip_ctx::kBranches);
+  std::vector<int> order(npairs), branch(npairs, 0);
+  std::vector<double> cost(npairs), load(nbr, 0.);
   for (int k = 0; k < npairs; ++k) {
-    size_t sz[kNumScratch];
-    scratch_sizes(pl[k], sz);
-    for (int i = 0; i < kNumScratch; ++i) need[i] = std::max(need[i], sz[i]);
+    const double n = (double)pl[k].N;
+    cost[k] = (double)pl[k].nx * pl[k].ny * (15. * n * n * std::log2(n) + 14. * n * n);
+    order[k] = k;
   }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  for (int k : order) {
+    int b = 0;
+    for (int q = 1; q < nbr; ++q)
+      if (load[q] < load[b]) b = q;
+    branch[k] = b;
+    load[b] += cost[k];
+  }
+  // scratch of each branch for its largest pair, counters per pair
   int rc;
-  for (int i = 0; i < kNumScratch; ++i)
-    if ((rc = ensure_b(c, c->bscr[i], need[i]))) return rc;
+  for (int b = 0; b < nbr; ++b) {
+    size_t need[kNumScratch];
+    for (int i = 0; i < kNumScratch; ++i) need[i] = 256;
+    for (int k = 0; k < npairs; ++k) {
+      if (branch[k] != b) continue;
+      size_t sz[kNumScratch];
+      scratch_sizes(pl[k], sz);
+      for (int i = 0; i < kNumScratch; ++i) need[i] = std::max(need[i], sz[i]);
+    }
+    for (int i = 0; i < kNumScratch; ++i)
+      if ((rc = ensure_b(c, c->bscr[b][i], need[i]))) return rc;
+    if (!c->bstream[b] && hipStreamCreateWithFlags(&c->bstream[b], hipStreamNonBlocking) != hipSuccess)
+      return FLCT_HIP_E_HIP;
+  }
+  for (auto& e : c->bev)
+    if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return FLCT_HIP_E_HIP;
   // transform tables of boxes above 96 (cached in the context before capture)
   std::vector<Tables> btab(npairs);
   for (int k = 0; k < npairs; ++k)
@@ -1374,20 +1412,25 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
   }
   if (!c->gexec || key != c->gkey) {
     drop_graph(c);
-    Scratch sc;
-    sc.list = (int*)c->bscr[0].p;
-    sc.hs1 = (double*)c->bscr[1].p;
-    sc.hs2 = (double*)c->bscr[2].p;
-    sc.mean1 = (double*)c->bscr[3].p;
-    sc.mean2 = (double*)c->bscr[4].p;
-    sc.flags = (int*)c->bscr[5].p;
-    sc.elist = (int*)c->bscr[6].p;
-    sc.planes = c->bscr[7].p;
     unsigned char* base = (unsigned char*)c->barena.p;
-    LaunchCtx lc{s, c->num_cu, c->lds_per_block};
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-    int erc = FLCT_HIP_OK;
-    for (int k = 0; k < npairs && !erc; ++k) {
+    // fork: every branch stream joins the capture through an event of s
+    int erc = (hipEventRecord(c->bev[0], s) == hipSuccess) ? FLCT_HIP_OK : FLCT_HIP_E_HIP;
+    for (int b = 0; b < nbr && !erc; ++b)
+      if (hipStreamWaitEvent(c->bstream[b], c->bev[0], 0) != hipSuccess) erc = FLCT_HIP_E_HIP;
+    for (int k : order) {
+      if (erc) break;
+      const int b = branch[k];
+      Scratch sc;
+      sc.list = (int*)c->bscr[b][0].p;
+      sc.hs1 = (double*)c->bscr[b][1].p;
+      sc.hs2 = (double*)c->bscr[b][2].p;
+      sc.mean1 = (double*)c->bscr[b][3].p;
+      sc.mean2 = (double*)c->bscr[b][4].p;
+      sc.flags = (int*)c->bscr[b][5].p;
+      sc.elist = (int*)c->bscr[b][6].p;
+      sc.planes = c->bscr[b][7].p;
+      LaunchCtx lc{c->bstream[b], c->num_cu, c->lds_per_block};
       Tables tb = btab[k];
       tb.gw = (const double*)(base + o_gw[k]);
       tb.filt = (o_filt[k] != (size_t)-1) ? (const double*)(base + o_filt[k]) : nullptr;
@@ -1395,6 +1438,12 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
       unsigned char* cnt = (unsigned char*)c->bcount.p + (size_t)k * 32;
       erc = enqueue_track(lc, pl[k], tb, sc, (unsigned int*)cnt, (unsigned long long*)(cnt + 16),
                           pairs[k].f1, pairs[k].f2, pairs[k].vx, pairs[k].vy, pairs[k].vm, nullptr);
+    }
+    // join: s waits for every branch before the capture ends
+    for (int b = 0; b < nbr; ++b) {
+      if (hipEventRecord(c->bev[1 + b], c->bstream[b]) != hipSuccess ||
+          hipStreamWaitEvent(s, c->bev[1 + b], 0) != hipSuccess)
+        if (!erc) erc = FLCT_HIP_E_HIP;
     }
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(s, &g);
