@@ -297,24 +297,19 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
         };
         if (t >= 1) {
           const int k = t, km = N - t;
-          cplx<T> u[M];
-          // A = (Zk + conj(Zm))/2, transformed and parked in row k
+          // A = (Zk + conj(Zm))/2 and B = (Zk - conj(Zm))/(2i) from one read
+          // of Phi rows k and N - k; both spectra stay in registers
+          cplx<T> u[M], v[M];
           static_for<M>([&](auto Rr) {
             const int jj = 4 * Rr + l;
             const cplx<T> zk = zval(k, jj), zm = zval(km, jj);
             u[Rr] = {T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
+            v[Rr] = {T(0.5) * (zk.im + zm.im), T(-0.5) * (zk.re - zm.re)};
           });
           quad_dit<M, -1>(u, l);
-          static_for<M>([&](auto S) { Z[zidx<M>(k, 4 * S + l)] = u[S]; });
-          // B = (Zk - conj(Zm))/(2i)
-          static_for<M>([&](auto Rr) {
-            const int jj = 4 * Rr + l;
-            const cplx<T> zk = zval(k, jj), zm = zval(km, jj);
-            u[Rr] = {T(0.5) * (zk.im + zm.im), T(-0.5) * (zk.re - zm.re)};
-          });
-          quad_dit<M, -1>(u, l);
+          quad_dit<M, -1>(v, l);
           static_for<M>([&](auto S) {
-            cplx<T> Aa = Z[zidx<M>(k, 4 * S + l)], B = u[S];
+            cplx<T> Aa = u[S], B = v[S];
             if (a.filt) {
               const T f = T(a.filt[k * N + lb + S]);
               Aa = {Aa.re * f, Aa.im * f};
