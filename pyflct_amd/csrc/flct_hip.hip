@@ -300,7 +300,8 @@ enum Path {
   PATH_BF16_SEG = 2,
   PATH_BF16_PIXEL = 3,
   PATH_FP64_PLANE = 4,
-  PATH_FP64_ROWSEG = 5  // FP64, shared row transforms (k_track_rs)
+  PATH_FP64_ROWSEG = 5,  // FP64, shared row transforms (k_track_rs)
+  PATH_FP32_ROWSEG = 6   // shared row transforms in FP64, row tasks in FP32 (k_track_rs)
 };
 constexpr int kQuadMaxN = 4 * kTrackMaxM;  // largest box of the quad-FFT kernels
 
@@ -312,6 +313,7 @@ static const char* path_name(int p) {
     case PATH_BF16_PIXEL: return "bf16-pixel";
     case PATH_FP64_PLANE: return "fp64-plane";
     case PATH_FP64_ROWSEG: return "fp64-rowseg";
+    case PATH_FP32_ROWSEG: return "fp32-rowseg";
     default: return "none";
   }
 }
@@ -346,27 +348,47 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
   } else {
     v = {PATH_FP64};
   }
-  if (rowseg_ok(N, bias, hires, skipon)) v.push_back(PATH_FP64_ROWSEG);
+  if (rowseg_ok(N, bias, hires, skipon)) {
+    if (precision != FLCT_HIP_FP64) v.push_back(PATH_FP32_ROWSEG);
+    v.push_back(PATH_FP64_ROWSEG);
+  }
   return v;
 }
 
 // Untuned choice: the split-BF16 request on boxes up to 20 runs the FP32
 // kernels (2048^2: 7.2 and 12.7 ms at boxes 16 and 20 against 14.5 and
-// 14.6 ms for the row-segment kernel, whose cost is flat below box 32).  On boxes above 64 it runs the FP32
-// kernels (with their FP64 re-evaluation of unsettled pixels), which are
-// within the split-BF16 error bound and measured 29-41 % faster than the
-// FP64 kernels and 2.9-3.7x faster than the dense-DFT BF16 kernel there
-// (2048^2, boxes 68..84).
-// Box sizes where the row-shared FP64 kernel was measured faster than
-// k_track by 4 % or more (2048^2 full-disk pair, every pixel tracked, device
-// ms k_track / k_track_rs): 8 3.03/2.60, 12 5.85/4.99, 32 31.1/29.7,
-// 36 56.2/42.9, 40 80.5/76.3, 48 125/114, 52 174/150, 60 202/172, 64 211/153,
-// 68 448/407, 72 443/356, 96 700/583.  Level or slower elsewhere (4, 16..28,
-// 44, 76..92).
+// 14.6 ms for the row-segment kernel, whose cost is flat below box 32).  On
+// boxes above 64 it runs the FP32 kernels (with their FP64 re-evaluation of
+// unsettled pixels), which are within the split-BF16 error bound and were
+// measured 29-41 % faster than the FP64 kernels and 2.9-3.7x faster than
+// the dense-DFT BF16 kernel there (2048^2, boxes 68..84).  Among the FP32
+// kernels the row-shared one is taken where it was faster (rowseg32_default).
+// Box sizes where a row-shared kernel was measured faster than the quad-FFT
+// kernel of its precision by 4 % or more (2048^2 full-disk pair, every pixel
+// tracked, whole-call ms, quad FFT / row-shared).
+// FP64: 8 3.1/2.5, 12 5.9/4.9, 16 7.7/7.1, 28 28.8/27.4, 32 31.2/26.8,
+// 36 56.5/39.6, 40 80.7/66.3, 48 125/109, 52 174/125, 60 203/140, 64 212/143,
+// 68 449/332, 72 442/335, 76 513/387, 80 497/370, 84 549/417, 88 589/496,
+// 96 709/539; level or slower at 4, 20, 24, 44 and 92.
 static bool rowseg_default(int N) {
   switch (N) {
-    case 8: case 12: case 32: case 36: case 40: case 48: case 52: case 60: case 64: case 68:
-    case 72: case 96:
+    case 8: case 12: case 16: case 28: case 32: case 36: case 40: case 48: case 52: case 56:
+    case 60: case 64: case 68: case 72: case 76: case 80: case 84: case 88: case 96:
+      return true;
+    default:
+      return false;
+  }
+}
+// FP32 row tasks (fp32-rowseg against fp32-fft): 8 1.9/2.9, 12 4.1/4.7,
+// 16 5.8/7.2, 32 25.4/30.5, 36 37.1/53.7, 40 46.3/59.1, 44 62.6/100.4,
+// 48 64.5/110.4, 52 95.9/148.8, 60 115.7/142.4, 64 113.9/165.4, 68 238/266,
+// 72 204/287, 76 284/328, 80 242/343, 84 302/382, 96 400/446; level or
+// slower at 4, 20..28, 88 and 92.  Used by the FP32 form and by the
+// split-BF16 form on the boxes it runs in FP32.
+static bool rowseg32_default(int N) {
+  switch (N) {
+    case 8: case 12: case 16: case 32: case 36: case 40: case 44: case 48: case 52: case 56:
+    case 60: case 64: case 68: case 72: case 76: case 80: case 84: case 96:
       return true;
     default:
       return false;
@@ -466,20 +488,23 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Pla
       pl->segpx = it->second.segpx;
     } else {
       pl->path = default_path(o->precision, N, nx, ny, o->bias, c->bf16_kernel);
-      // FP64 without a threshold: the row-shared kernel where it was faster
-      if (pl->path == PATH_FP64 && o->thresh == 0. && rowseg_default(N) &&
-          rowseg_ok(N, o->bias, o->hires, pl->r.skipon))
-        pl->path = PATH_FP64_ROWSEG;
+      // without a threshold: the row-shared kernels where they were faster
+      // (with one, a tracked-pixel list serves sparse selections better)
+      if (o->thresh == 0. && rowseg_ok(N, o->bias, o->hires, pl->r.skipon)) {
+        if (pl->path == PATH_FP64 && rowseg_default(N)) pl->path = PATH_FP64_ROWSEG;
+        if (pl->path == PATH_FP32 && rowseg32_default(N)) pl->path = PATH_FP32_ROWSEG;
+      }
     }
   }
   if (pl->path == PATH_BF16_SEG && !seg_supported(N, nx, ny, o->bias)) pl->path = PATH_BF16_PIXEL;
-  if (pl->path == PATH_FP64_ROWSEG) {
+  if (pl->path == PATH_FP64_ROWSEG || pl->path == PATH_FP32_ROWSEG) {
     if (rowseg_ok(N, o->bias, o->hires, pl->r.skipon)) {
       const int w = (pl->segpx >= 8 && pl->segpx < kRsSeg) ? pl->segpx : kRsSeg;
       pl->rs_nband = (ny + w - 1) / w;
-      pl->rs_grid = rs_grid(N / 4, c->num_cu, (long long)nx * pl->rs_nband, pl->slots);
+      pl->rs_grid = rs_grid(N / 4, c->num_cu, (long long)nx * pl->rs_nband, pl->slots,
+                            pl->path == PATH_FP32_ROWSEG);
     } else {
-      pl->path = PATH_FP64;
+      pl->path = (pl->path == PATH_FP32_ROWSEG) ? PATH_FP32 : PATH_FP64;
     }
   }
   if (o->hires) pl->path = PATH_FP64;  // -h: the FP64 kernels carry the hires refinement
@@ -535,7 +560,8 @@ void scratch_sizes(const Plan& pl, size_t sz[kNumScratch]) {
   sz[5] = (pl.path != PATH_FP64) ? npix * sizeof(int) : 0;
   sz[6] = pl.seg ? (pl.sparse ? npix : (long long)pl.nx * (N - 1)) * sizeof(int) : 0;
   sz[7] = (pl.path == PATH_FP64_PLANE)    ? (size_t)pl.big_grid * 24 * (size_t)N * N
-          : (pl.path == PATH_FP64_ROWSEG) ? (size_t)pl.rs_grid * 16 * N * (size_t)(kRsSeg + N)
+          : (pl.path == PATH_FP64_ROWSEG || pl.path == PATH_FP32_ROWSEG)
+              ? (size_t)pl.rs_grid * 16 * N * (size_t)(kRsSeg + N)
                                           : 0;
 }
 
@@ -616,7 +642,8 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
   ta.slots = pl.slots;
   // the split-BF16 and FP32 kernels list the pixels their surface cannot
   // settle; those are tracked again by the FP64 kernels
-  const bool refine = bf16 || pl.path == PATH_FP32 || pl.path == PATH_FP64_ROWSEG;
+  const bool refine = bf16 || pl.path == PATH_FP32 || pl.path == PATH_FP64_ROWSEG ||
+                      pl.path == PATH_FP32_ROWSEG;
   ta.flag_list = refine ? sc.flags : nullptr;
   ta.flag_count = refine ? counts + 1 : nullptr;
   if (ev) HIPCHK(hipEventRecord(ev[1], s));
@@ -650,7 +677,7 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
     ba.fwd = tb.big_fwd;
     ba.inv = tb.big_inv;
     if ((rc = launch_track_big(lc, ba, pl.big_grid))) return rc;
-  } else if (pl.path == PATH_FP64_ROWSEG) {
+  } else if (pl.path == PATH_FP64_ROWSEG || pl.path == PATH_FP32_ROWSEG) {
     RsArgs ra;
     ra.t = ta;
     ra.thresh = o->thresh;
@@ -659,7 +686,7 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
     ra.nband = pl.rs_nband;
     ra.ustride = kRsSeg + N;
     ra.phi = (double2*)sc.planes;
-    if ((rc = dispatch_track_rs(M, lc, ra, pl.rs_grid))) return rc;
+    if ((rc = dispatch_track_rs(M, lc, ra, pl.rs_grid, pl.path == PATH_FP32_ROWSEG))) return rc;
   } else if (bf16) {
     if (pl.seg) {
       long long nedge = 0;
@@ -981,7 +1008,8 @@ static int end_call(flct_hip_ctx* c, const Plan& pl, const unsigned int* counts)
   HIPCHK(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   const bool refine = (pl.path == PATH_FP32 || pl.path == PATH_BF16_SEG ||
-                       pl.path == PATH_BF16_PIXEL || pl.path == PATH_FP64_ROWSEG);
+                       pl.path == PATH_BF16_PIXEL || pl.path == PATH_FP64_ROWSEG ||
+                       pl.path == PATH_FP32_ROWSEG);
   c->last_path = pl.path;
   c->last_slots = pl.slots;
   c->last_segpx = pl.segpx;
@@ -1161,11 +1189,14 @@ static int path_slots(int path, int N) {
   if (N % 4 != 0 || M < 1 || M > kTrackMaxM) return 0;
   if (path == PATH_FP64) return track_slots<double>(M);
   if (path == PATH_FP32) return track_slots<float>(M);
-  if (path == PATH_FP64_ROWSEG) return rs_slots(M);
+  if (path == PATH_FP64_ROWSEG) return rs_slots(M, false);
+  if (path == PATH_FP32_ROWSEG) return rs_slots(M, true);
   return 0;
 }
 
-static bool path_has_segpx(int path) { return path == PATH_BF16_SEG || path == PATH_FP64_ROWSEG; }
+static bool path_has_segpx(int path) {
+  return path == PATH_BF16_SEG || path == PATH_FP64_ROWSEG || path == PATH_FP32_ROWSEG;
+}
 
 // a runtime shape the path accepts: slots 0 or 1..its compiled count,
 // segpx 0 or 8..64 on the row-segment paths
@@ -1189,7 +1220,7 @@ extern "C" int flct_hip_set_config(flct_hip_ctx* c, int precision, int nbox, con
     return FLCT_HIP_OK;
   }
   int p = -1;
-  for (int q = PATH_FP64; q <= PATH_FP64_ROWSEG; ++q)
+  for (int q = PATH_FP64; q <= PATH_FP32_ROWSEG; ++q)
     if (std::strcmp(path, path_name(q)) == 0) p = q;
   if (p < 0) return FLCT_HIP_E_ARG;
   // the path must meet the form's error contract; the row-segment kernel
@@ -1471,7 +1502,8 @@ extern "C" int flct_hip_track_batch(flct_hip_ctx* c, flct_hip_pair* pairs, int n
     pairs[k].tracked = pl[k].sparse ? (long long)ck[0] : (long long)pl[k].nx * pl[k].ny;
     const int pk = pl[k].path;
     pairs[k].refined =
-        (pk == PATH_FP32 || pk == PATH_BF16_SEG || pk == PATH_BF16_PIXEL || pk == PATH_FP64_ROWSEG)
+        (pk == PATH_FP32 || pk == PATH_BF16_SEG || pk == PATH_BF16_PIXEL || pk == PATH_FP64_ROWSEG ||
+         pk == PATH_FP32_ROWSEG)
             ? (long long)ck[1]
             : 0;
     tracked += pairs[k].tracked;

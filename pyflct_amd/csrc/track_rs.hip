@@ -54,21 +54,31 @@ __device__ __forceinline__ double key_value(unsigned long long k) {
   return __builtin_bit_cast(double, b);
 }
 
-template <int M>
+// FP32 row tasks: pixels whose surface the FP32 arithmetic cannot settle
+// are listed for FP64 re-evaluation, as in the FP32 tracker (a second cell
+// within kTieTol32 of the maximum, or an ill-conditioned fit)
+template <int M, class T = double>
 struct RsCfg {
   static constexpr int N = 4 * M;
   static constexpr int kSlotThreads = 2 * N;
-  static constexpr int kRow = TrackCfg<M, double>::kRow;
-  static constexpr int kZBytes = ((N / 2 + 1) * kRow + 8) * 16;
-  static constexpr int kSlotBytes = ((kZBytes + N * 8 + 9 * 8 + 2 * 8 + 16 + 32) + 15) / 16 * 16;
+  static constexpr int kRow = TrackCfg<M, T>::kRow;
+  static constexpr int kZBytes = ((N / 2 + 1) * kRow + 8) * int(sizeof(cplx<T>));
+  // plane, row window, 9 neighbours, result, argmax words, box extrema,
+  // near-maximum count
+  static constexpr int kSlotBytes = ((kZBytes + N * 8 + 9 * 8 + 2 * 8 + 16 + 32 + 16) + 15) / 16 * 16;
   static constexpr int kColStat = kRsSeg + N;  // columns of a segment's boxes, at most
   static constexpr int kShared = 2 * N * 8 + N * 16 + kRsSeg * 4 + 16 + 4 * kColStat * 8;
-  // boxes up to 36 fit two waves per SIMD in registers (at most 255
+  // FP64: boxes up to 36 fit two waves per SIMD in registers (at most 255
   // VGPRs); larger boxes need up to ~500 registers per lane and run one
   // wave per SIMD in blocks of at most 256 threads, so that the allocator
-  // may use the accumulation registers as well
-  static constexpr int kMinWaves = (M <= 9) ? 2 : 1;
-  static constexpr int kMaxThreads = (M <= 9) ? 512 : 256;
+  // may use the accumulation registers as well.  FP32 row tasks always run
+  // two waves per SIMD: up to box 40 without spills, above it spilling 38
+  // to ~120 registers, which measured 1.5-2.1x faster than one wave with
+  // none (4096^2 box 64: 454 against 685 ms; 2048^2 box 52: 96 against
+  // 148 ms, box 80: 242 against 497 ms)
+  static constexpr bool kTwo = (M <= 9) || (sizeof(T) == 4);
+  static constexpr int kMinWaves = kTwo ? 2 : 1;
+  static constexpr int kMaxThreads = kTwo ? 512 : 256;
   static constexpr int fit() {
     int p = 1;
     while (p < 8 && (p + 1) * kSlotBytes + kShared <= 160 * 1024 &&
@@ -81,10 +91,9 @@ struct RsCfg {
   static constexpr int kLdsBytes = P * kSlotBytes + kShared;
 };
 
-template <int M>
-__global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_track_rs(RsArgs A) {
-  using Cfg = RsCfg<M>;
-  using T = double;
+template <int M, class T>
+__global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWaves)) k_track_rs(RsArgs A) {
+  using Cfg = RsCfg<M, T>;
   constexpr int N = 4 * M;
   extern __shared__ __align__(16) unsigned char lds[];
   const int tid = threadIdx.x;
@@ -94,7 +103,7 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
   const bool in_slot = slot < SP;
   const int lane = stid & 3, qd = stid >> 2;
   double* gw = reinterpret_cast<double*>(lds);                     // 2N - 1
-  cplx<T>* wrh = reinterpret_cast<cplx<T>*>(gw + 2 * N);           // N: Wr^(k)
+  cplx<double>* wrh = reinterpret_cast<cplx<double>*>(gw + 2 * N);  // N: Wr^(k)
   int* slist = reinterpret_cast<int*>(wrh + N);                    // kRsSeg
   int* scount = slist + kRsSeg;
   double* colst = reinterpret_cast<double*>(scount + 4);  // [4][kColStat]: min f1, max f1, min f2, max f2
@@ -107,6 +116,7 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
   unsigned long long* amax = reinterpret_cast<unsigned long long*>(res + 2);
   unsigned int* aidx = reinterpret_cast<unsigned int*>(amax + 1);
   unsigned long long* bst = amax + 2;  // box min f1, max f1, min f2, max f2 (order keys)
+  unsigned int* ntie = reinterpret_cast<unsigned int*>(bst + 4);  // FP32: near-maximum cells
   const TrackArgs& a = A.t;
   for (int t = tid; t < 2 * N - 1; t += blockDim.x) gw[t] = a.gw[t];
 
@@ -176,7 +186,7 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
       const double* p2 = a.f2 + (long long)imin * ny + cb;
 #pragma unroll 1
       for (int u = tid >> 2; u <= U; u += nq) {
-        cplx<T> z[M];
+        cplx<double> z[M];
         double mn1 = 0., mx1 = 0., mn2 = 0., mx2 = 0.;
         static_for<M>([&](auto Rr) {
           const int x = 4 * Rr + l;
@@ -246,6 +256,7 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
           bst[1] = 0ull;
           bst[2] = ~0ull;
           bst[3] = 0ull;
+          *ntie = 0u;
         }
       }
       __syncthreads();
@@ -271,6 +282,7 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
       }
       __syncthreads();
       bool zero = false, exact = false;  // zero surface; re-evaluate in k_track
+      double s1 = 1., s2 = 1.;             // FP32: image scales
       if (active) {
         const double mn1 = key_value(bst[0]), mx1 = key_value(bst[1]);
         const double mn2 = key_value(bst[2]), mx2 = key_value(bst[3]);
@@ -278,7 +290,27 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
         const double o1 = fmax(fabs(mx1 - mref1), fabs(mn1 - mref1));
         const double o2 = fmax(fabs(mx2 - mref2), fabs(mn2 - mref2));
         exact = !zero && (!(mx1 - mn1 > kRsRange * o1) || !(mx2 - mn2 > kRsRange * o2));
+        if constexpr (sizeof(T) == 4) {
+          // powers of two bringing each image's largest |f - fbar| (a bound
+          // of its windowed values) to [1, 2): exact, the argmax and the fit
+          // are scale-invariant, and the row tasks round each image to FP32
+          // at its own magnitude
+          const double g1 = fmax(fabs(mx1 - f1b), fabs(mn1 - f1b));
+          const double g2 = fmax(fabs(mx2 - f2b), fabs(mn2 - f2b));
+          if (g1 > 0.) s1 = ldexp(1.0, -ilogb(g1));
+          if (g2 > 0.) s2 = ldexp(1.0, -ilogb(g2));
+        }
       }
+      // A double of image X (1, 2) as a row-task value: FP64 unchanged, FP32
+      // scaled and rounded once
+      auto tv = [&](double x, double sc) -> T {
+        if constexpr (sizeof(T) == 8) {
+          (void)sc;
+          return x;
+        } else {
+          return (T)(x * sc);
+        }
+      };
       const bool compute = active && !zero && !exact;
       // ------------- phase 2: row transforms of A and B, cross power,
       // inverse along jj; rows k = 0..N/2 of the result stay in LDS
@@ -287,10 +319,10 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
         const int t = quad;
         const int lb = M * brev2(l);
         const int c0 = jmin - cb;
-        // Z(k, c) = Wc(c) (Phi(k, c0 + c) - (d1 + i d2) Wr^(k))
-        auto zval = [&](int k, int c) -> cplx<T> {
+        // Z(k, c) = Wc(c) (Phi(k, c0 + c) - (d1 + i d2) Wr^(k)), in FP64
+        auto zval = [&](int k, int c) -> cplx<double> {
           const double2 f = phi[(size_t)k * us + c0 + c];
-          const cplx<T> w = wrh[k];
+          const cplx<double> w = wrh[k];
           const double cr = d1 * w.re - d2 * w.im, ci = d1 * w.im + d2 * w.re;
           const double wc = wy[c];
           return {wc * (f.x - cr), wc * (f.y - ci)};
@@ -302,9 +334,9 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
           cplx<T> u[M], v[M];
           static_for<M>([&](auto Rr) {
             const int jj = 4 * Rr + l;
-            const cplx<T> zk = zval(k, jj), zm = zval(km, jj);
-            u[Rr] = {T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
-            v[Rr] = {T(0.5) * (zk.im + zm.im), T(-0.5) * (zk.re - zm.re)};
+            const cplx<double> zk = zval(k, jj), zm = zval(km, jj);
+            u[Rr] = {tv(0.5 * (zk.re + zm.re), s1), tv(0.5 * (zk.im - zm.im), s1)};
+            v[Rr] = {tv(0.5 * (zk.im + zm.im), s2), tv(-0.5 * (zk.re - zm.re), s2)};
           });
           quad_dit<M, -1>(u, l);
           quad_dit<M, -1>(v, l);
@@ -325,9 +357,9 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
           cplx<T> u[M], v[M];
           static_for<M>([&](auto Rr) {
             const int jj = 4 * Rr + l;
-            const cplx<T> z0 = zval(0, jj), zh = zval(N / 2, jj);
-            u[Rr] = {z0.re, zh.re};
-            v[Rr] = {z0.im, zh.im};
+            const cplx<double> z0 = zval(0, jj), zh = zval(N / 2, jj);
+            u[Rr] = {tv(z0.re, s1), tv(zh.re, s1)};
+            v[Rr] = {tv(z0.im, s2), tv(zh.im, s2)};
           });
           quad_dit<M, -1>(u, l);
           wave_lds_order();
@@ -413,6 +445,18 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
         best = min(best, (unsigned)__builtin_amdgcn_mov_dpp((int)best, kQuadXor1, 0xF, 0xF, true));
         best = min(best, (unsigned)__builtin_amdgcn_mov_dpp((int)best, kQuadXor2, 0xF, 0xF, true));
         if (l == 0 && best != 0xffffffffu) atomicMin(aidx, best);
+        if constexpr (sizeof(T) == 4) {
+          // cells within kTieTol32 of the maximum (one is the maximum itself)
+          if (gmax != 0.) {
+            const double thr = gmax * (1. - kTieTol32);
+            unsigned cnt = 0;
+            static_for<M>([&](auto S) {
+              cnt += ((double)va[S] >= thr) ? 1u : 0u;
+              cnt += ((double)vb[S] >= thr) ? 1u : 0u;
+            });
+            if (cnt) atomicAdd(ntie, cnt);
+          }
+        }
       }
       __syncthreads();
       const unsigned loc = *aidx;
@@ -438,6 +482,7 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
       if (active && stid == 0) {
 #pragma clang fp contract(off)
         double shiftsubx = 0., shiftsuby = 0.;
+        bool flagged = false;
         if (fit) {
           // nb[(dx+1)*3 + (dy+1)] = surface at (ixmax+dx, iymax+dy)
           const double c00 = nb[4];
@@ -448,6 +493,18 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
           const double fxy = 0.25 * (nb[8] + nb[0] - nb[6] - nb[2]);
           shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
           shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
+          if constexpr (sizeof(T) == 4) {
+            // first-order response of the fit to relative errors kRelErr32
+            // of the nine values
+            const double det = fxx * fyy - fxy * fxy;
+            const double hn = fabs(fxx) + fabs(fyy) + 2. * fabs(fxy);
+            const double bound =
+                kRelErr32 * gmax * hn * (1. + 4. * (fabs(shiftsubx) + fabs(shiftsuby)));
+            if (!(bound <= kFitTol32 * fabs(det))) flagged = true;
+          }
+        }
+        if constexpr (sizeof(T) == 4) {
+          if (nonzero && *ntie > 1u) flagged = true;
         }
         const double shiftx = ((double)ixmax + shiftsubx) - (double)(N / 2);
         const double shifty = ((double)iymax + shiftsuby) - (double)(N / 2);
@@ -458,6 +515,7 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
           a.vx[p] = shifty * a.deltinv * a.deltas;  // transp = 1 (flct.c:901-905)
           a.vy[p] = shiftx * a.deltinv * a.deltas;
           a.vm[p] = 1.;
+          if (flagged) a.flag_list[atomicAdd(a.flag_count, 1u)] = (int)p;  // FP32: re-evaluated
         }
       }
       __syncthreads();  // Z, nb and the maxima are rewritten by the next pixel
@@ -466,57 +524,62 @@ __global__ void __launch_bounds__(RsCfg<M>::kThreads, RsCfg<M>::kMinWaves) k_tra
 }
 
 // LDS bytes and threads of a block of sp slots (1..P)
-template <int M>
+template <int M, class T>
 static void rs_block(int sp, size_t* lds, int* threads) {
-  using Cfg = RsCfg<M>;
+  using Cfg = RsCfg<M, T>;
   *lds = (size_t)sp * Cfg::kSlotBytes + Cfg::kShared;
   *threads = (sp * Cfg::kSlotThreads + 63) / 64 * 64;
 }
 
-template <int M>
+template <int M, class T>
 static int launch_rs(const LaunchCtx& c, const RsArgs& a, long long grid) {
   size_t lds = 0;
   int threads = 0;
-  rs_block<M>(run_slots(a.t.slots, RsCfg<M>::P), &lds, &threads);
+  rs_block<M, T>(run_slots(a.t.slots, RsCfg<M, T>::P), &lds, &threads);
   if (lds > c.lds_per_block) return FLCT_HIP_E_UNSUPPORTED;
   if (lds > 65536)
-    (void)hipFuncSetAttribute((const void*)k_track_rs<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    (void)hipFuncSetAttribute((const void*)k_track_rs<M, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
-  hipLaunchKernelGGL(k_track_rs<M>, dim3((unsigned)grid), dim3(threads), lds, c.stream, a);
+  hipLaunchKernelGGL((k_track_rs<M, T>), dim3((unsigned)grid), dim3(threads), lds, c.stream, a);
   return hipGetLastError() == hipSuccess ? FLCT_HIP_OK : FLCT_HIP_E_HIP;
 }
 
-long long rs_grid(int M, int num_cu, long long nseg, int slots) {
+template <class T>
+static long long rs_grid_t(int M, int num_cu, long long nseg, int slots) {
   size_t lds = 0;
   int threads = 0;
   static_for<kTrackMaxM>([&](auto I) {
     constexpr int m = I + 1;
-    if (m == M) rs_block<m>(run_slots(slots, RsCfg<m>::P), &lds, &threads);
+    if (m == M) rs_block<m, T>(run_slots(slots, RsCfg<m, T>::P), &lds, &threads);
   });
   if (lds == 0) return 0;
   long long per_cu = (160LL * 1024) / (long long)lds;
   if (per_cu > 2048 / threads) per_cu = 2048 / threads;
   if (per_cu < 1) per_cu = 1;
-  if (per_cu > 4) per_cu = 4;
+  if (per_cu > (sizeof(T) == 4 ? 8 : 4)) per_cu = (sizeof(T) == 4 ? 8 : 4);
   long long g = (long long)num_cu * per_cu;
   if (g > nseg) g = nseg;
   return g < 1 ? 1 : g;
 }
 
-int rs_slots(int M) {
+long long rs_grid(int M, int num_cu, long long nseg, int slots, bool fp32) {
+  return fp32 ? rs_grid_t<float>(M, num_cu, nseg, slots) : rs_grid_t<double>(M, num_cu, nseg, slots);
+}
+
+int rs_slots(int M, bool fp32) {
   int p = 0;
   static_for<kTrackMaxM>([&](auto I) {
     constexpr int m = I + 1;
-    if (m == M) p = RsCfg<m>::P;
+    if (m == M) p = fp32 ? RsCfg<m, float>::P : RsCfg<m, double>::P;
   });
   return p;
 }
 
-int dispatch_track_rs(int M, const LaunchCtx& c, const RsArgs& a, long long grid) {
+int dispatch_track_rs(int M, const LaunchCtx& c, const RsArgs& a, long long grid, bool fp32) {
   int rc = FLCT_HIP_E_UNSUPPORTED;
   static_for<kTrackMaxM>([&](auto I) {
     constexpr int m = I + 1;
-    if (m == M) rc = launch_rs<m>(c, a, grid);
+    if (m == M) rc = fp32 ? launch_rs<m, float>(c, a, grid) : launch_rs<m, double>(c, a, grid);
   });
   return rc;
 }

@@ -20,10 +20,11 @@ struct RsArgs {
 };
 
 // blocks of the persistent grid for box size 4*M with `slots` pixels per
-// block (0: the compiled count; 0 blocks: no such box)
-long long rs_grid(int M, int num_cu, long long nseg, int slots);
+// block (0: the compiled count; 0 blocks: no such box); fp32: the row tasks
+// in FP32 (path fp32-rowseg), else FP64
+long long rs_grid(int M, int num_cu, long long nseg, int slots, bool fp32);
 // compiled slots per block of k_track_rs for box size 4*M
-int rs_slots(int M);
-int dispatch_track_rs(int M, const LaunchCtx& c, const RsArgs& a, long long grid);
+int rs_slots(int M, bool fp32);
+int dispatch_track_rs(int M, const LaunchCtx& c, const RsArgs& a, long long grid, bool fp32);
 
 }  // namespace flct

@@ -207,7 +207,7 @@ def test_paths_and_autotune():
         c.track(f1, f2, 1.0, 1.0, 15.0, precision="split_bf16")
         assert c.last_path() == "bf16-segment"
         b84 = c.track(f1, f2, 1.0, 1.0, 20.0, precision="split_bf16")  # box 84
-        assert c.last_path() == "fp32-fft"
+        assert c.last_path() == "fp32-rowseg"
         r84 = c.track(f1, f2, 1.0, 1.0, 20.0)
         assert np.array_equal(b84[2], r84[2])
         rms, mx = _errs(r84, b84)
@@ -217,8 +217,8 @@ def test_paths_and_autotune():
             c.upload(d[0], f1)
             c.upload(d[1], f2)
             for form, sigma, allowed in (("split_bf16", 15.0, {"bf16-segment", "bf16-pixel", "fp32-fft",
-                                                               "fp64-fft", "fp64-rowseg"}),
-                                         ("fp32", 9.0, {"fp32-fft", "fp64-fft", "fp64-rowseg"}),
+                                                               "fp32-rowseg", "fp64-fft", "fp64-rowseg"}),
+                                         ("fp32", 9.0, {"fp32-fft", "fp32-rowseg", "fp64-fft", "fp64-rowseg"}),
                                          ("fp64", 12.0, {"fp64-fft", "fp64-rowseg"})):
                 best = c.autotune(d[0], d[1], 384, 384, c.make_opts(1.0, 1.0, sigma, precision=form))
                 assert best in allowed, (form, best)
@@ -248,6 +248,9 @@ def test_paths_and_autotune():
 # paths: FP32 kernels for every box in FP32; in split-BF16 the FP32 kernels up
 # to box 20 and above 64, the row-segment kernel from 24 to 64)
 BOXES = list(range(4, 100, 4))
+# boxes whose untuned FP32 kernel (FP32 form; split-BF16 outside 24..64) is
+# the row-shared one (fp32-rowseg) when no threshold is given
+FP32_ROWSEG_BOXES = {8, 12, 16, 32, 36, 40, 44, 48, 52, 56, 60, 64, 68, 72, 76, 80, 84, 96}
 
 
 def _box_case(nbox):
@@ -295,7 +298,8 @@ def test_pixel_kernels_every_box_size(monkeypatch):
             a, b = smooth_pair(rng, nx, ny, shift=(-0.7, 0.4), width=max(1.5, nbox / 16))
             ref = c.track(a, b, 1.0, 1.0, sigma)
             got = c.track(a, b, 1.0, 1.0, sigma, precision="split_bf16")
-            assert c.last_path() == ("bf16-pixel" if nbox <= 72 else "fp32-fft"), (nbox, c.last_path())
+            fp32_path = "fp32-rowseg" if nbox in FP32_ROWSEG_BOXES else "fp32-fft"
+            assert c.last_path() == ("bf16-pixel" if nbox <= 72 else fp32_path), (nbox, c.last_path())
             assert np.array_equal(got[2], ref[2]), nbox
             rms, mx = _errs(ref, got)
             assert rms <= RMS_TOL and mx <= MAX_TOL, (nbox, rms, mx)

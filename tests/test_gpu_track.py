@@ -399,18 +399,23 @@ def test_hires_sigma0(ctx, nx, ny, kw):
     assert_parity(g, o, where=f"hires sigma=0 {nx}x{ny}")
 
 
-def test_fp32_form_refines_unsettled_pixels(ctx):
+@pytest.mark.parametrize("path", ["fp32-fft", "fp32-rowseg"])
+def test_fp32_form_refines_unsettled_pixels(ctx, path):
     """The FP32 form lists pixels whose FP32 surface cannot settle the peak
     (a second cell within 1e-5 of the maximum, or an ill-conditioned fit)
     and tracks them again in FP64, which bounds its worst error on a
     full-disk pair with a zero limb (boxes grazing the limb are the hard
-    ones)."""
+    ones).  Both FP32 kernels: the quad-FFT tracker and the row-shared one."""
     from pyflct_amd import synth
     f1, f2, _ = synth.cfg4(n=512)
     ref = ctx.track(f1, f2, 1.0, 1.0, 8.0)
-    g = ctx.track(f1, f2, 1.0, 1.0, 8.0, precision="fp32")
-    st = ctx.last_stats()
-    assert ctx.last_path() == "fp32-fft" and st["refined"] > 0
+    ctx.set_path("fp32", 32, path)
+    try:
+        g = ctx.track(f1, f2, 1.0, 1.0, 8.0, precision="fp32")
+        st = ctx.last_stats()
+        assert ctx.last_path() == path and st["refined"] > 0
+    finally:
+        ctx.set_path("fp32", 32, None)
     assert np.array_equal(g[2], ref[2])
     err = np.hypot(g[0] - ref[0], g[1] - ref[1])
     assert np.sqrt(np.mean(err ** 2)) <= 1e-4 and err.max() <= 1e-2, (err.max(), st["refined"])
