@@ -367,12 +367,12 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
 // kernel of its precision by 4 % or more (2048^2 full-disk pair, every pixel
 // tracked, whole-call ms, quad FFT / row-shared).
 // FP64: 8 3.1/2.5, 12 5.9/4.9, 16 7.7/7.1, 28 28.8/27.4, 32 31.2/26.8,
-// 36 56.5/39.6, 40 80.7/66.3, 48 125/109, 52 174/125, 60 203/140, 64 212/143,
-// 68 449/332, 72 442/335, 76 513/387, 80 497/370, 84 549/417, 88 589/496,
-// 96 709/539; level or slower at 4, 20, 24, 44 and 92.
+// 36 56.5/39.6, 40 80.7/48.7, 44 106/72.2, 48 125/79.2, 52 174/125,
+// 60 203/140, 64 212/143, 68 449/332, 72 442/335, 76 513/387, 80 497/370,
+// 84 549/417, 88 589/496, 96 709/539; level or slower at 4, 20, 24 and 92.
 static bool rowseg_default(int N) {
   switch (N) {
-    case 8: case 12: case 16: case 28: case 32: case 36: case 40: case 48: case 52: case 56:
+    case 8: case 12: case 16: case 28: case 32: case 36: case 40: case 44: case 48: case 52: case 56:
     case 60: case 64: case 68: case 72: case 76: case 80: case 84: case 88: case 96:
       return true;
     default:

@@ -69,14 +69,17 @@ struct RsCfg {
   static constexpr int kColStat = kRsSeg + N;  // columns of a segment's boxes, at most
   static constexpr int kShared = 2 * N * 8 + N * 16 + kRsSeg * 4 + 16 + 4 * kColStat * 8;
   // FP64: boxes up to 36 fit two waves per SIMD in registers (at most 255
-  // VGPRs); larger boxes need up to ~500 registers per lane and run one
-  // wave per SIMD in blocks of at most 256 threads, so that the allocator
-  // may use the accumulation registers as well.  FP32 row tasks always run
+  // VGPRs); 40..48 run two waves with spills, which measured 27-32 % faster
+  // than one wave (2048^2: 48.7, 72.2, 79.2 against 66.3, 105.6, 108.9 ms);
+  // larger boxes need up to ~500 registers per lane and run one wave per
+  // SIMD in blocks of at most 256 threads, so that the allocator may use the
+  // accumulation registers as well (two waves were slower from box 52 up:
+  // box 64 227 against 143 ms).  FP32 row tasks always run
   // two waves per SIMD: up to box 40 without spills, above it spilling 38
   // to ~120 registers, which measured 1.5-2.1x faster than one wave with
   // none (4096^2 box 64: 454 against 685 ms; 2048^2 box 52: 96 against
   // 148 ms, box 80: 242 against 497 ms)
-  static constexpr bool kTwo = (M <= 9) || (sizeof(T) == 4);
+  static constexpr bool kTwo = (M <= 12) || (sizeof(T) == 4);
   static constexpr int kMinWaves = kTwo ? 2 : 1;
   static constexpr int kMaxThreads = kTwo ? 512 : 256;
   static constexpr int fit() {
