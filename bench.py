@@ -206,7 +206,7 @@ def main():
                          "px_per_s": st["tracked"] / (ms * 1e-3),
                          "kernel_ms": float(np.mean(kms)), "main_ms": float(np.mean(mms)),
                          "refine_ms": float(np.mean(rms_)), "refined": st["refined"],
-                         "vx": vx, "vy": vy}
+                         "path": ctx.last_path(), "vx": vx, "vy": vy}
         if form == "fp64":
             ref = (vx, vy)
         log(f"bench: {form}: {ms:.1f} ms/step, kernel {np.mean(kms):.1f} ms, "
@@ -230,7 +230,7 @@ def main():
     for form, r in results.items():
         counted = pmc.get(form, {}).get("lds_bytes_per_launch")
         model = None
-        if form in ("fp64", "fp32"):
+        if r["path"] in ("fp64-fft", "fp32-fft"):  # the quad-FFT tracker's access pattern
             model = lds_bytes_fft(nbox, 16 if form == "fp64" else 8,
                                   2 if (form == "fp32" and 44 <= nbox <= 96) else 1) * r["tracked"]
         b = counted if counted else model
@@ -344,7 +344,8 @@ def main():
         "config": {"workload": f"full-disk pair {nx}x{ny}, sigma={args.sigma:g} "
                                f"({nbox}x{nbox} boxes), all pixels tracked",
                    "form": head, "box": nbox, "tracked_px": hr["tracked"]},
-        "forms": {f: {"px_per_s": round(r["px_per_s"], 1), "ms_per_step": round(r["ms_per_step"], 3),
+        "forms": {f: {"path": r["path"], "px_per_s": round(r["px_per_s"], 1),
+                      "ms_per_step": round(r["ms_per_step"], 3),
                       "kernel_ms": round(r["kernel_ms"], 3), "main_ms": round(r["main_ms"], 3),
                       "refine_ms": round(r["refine_ms"], 3), "refined_px": r["refined"],
                       "rms_vel_err_vs_fp64": r["rms_vel_err_vs_fp64"],
