@@ -140,8 +140,9 @@ typedef struct flct_hip_pair {
  * kernels are captured into one HIP graph on the context's stream and
  * launched once.  Inside the graph the pairs run on up to four concurrent
  * branches (largest estimated work first, each pair to the least loaded
- * branch), each branch with its own scratch; output arrays of different
- * pairs must not overlap.  The graph is kept and relaunched while the next call
+ * branch), each branch with its own scratch; so pairs run in no fixed
+ * order, and a pair's output arrays must not overlap any other pair's input
+ * or output arrays.  The graph is kept and relaunched while the next call
  * brings the same pairs (arrays, sizes, options).  Every pair is validated
  * first: if any is invalid, nothing runs, each pair's status is set and the
  * first error is returned.  flct_hip_last_stats reports the sum of the
