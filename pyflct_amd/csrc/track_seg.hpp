@@ -689,6 +689,11 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
                   const unsigned long long m = __ballot(fabsf(cv[NT][q][IT][pm][int(t)]) >= thr);
                   // branch-free: a zero mask leaves nge and loc as they are
                   nge += __popcll(m);
+                  // counted here: left to the compiler, the 64 counts sink to
+                  // the end and every mask stays live in scalar registers
+                  // (128 of them, spilled to vector lanes: 108 readlane and
+                  // writelane instructions per pixel)
+                  asm volatile("" : "+s"(nge));
                   const int ln = __builtin_ctzll(m | (1ull << 63));
                   const int ii = 16 * IT + (ln & 15);
                   const int sx = (pm == 0) ? ii + Nh : ii;
