@@ -107,3 +107,33 @@ def test_split_bf16_may_run_it(ctx32):
     assert np.array_equal(got[2], ref[2])
     rms, mx = _errs(ref, got)
     assert rms <= 1e-4 and mx <= 1e-2, (rms, mx)
+
+
+def test_filter_and_batch(ctx32):
+    """-k (the filter applied to both spectra of every row task) and the
+    batch graph: each pair of a batch equals its single call bit for bit."""
+    c = ctx32
+    rng = np.random.default_rng(41)
+    items = []
+    for nx, ny, sigma, kw in ((300, 260, 15.0, {"kr": 0.5}), (200, 230, 9.0, {}),
+                              (260, 300, 19.0, {"kr": 0.7})):
+        a, b = smooth_pair(rng, nx, ny, shift=(0.5, -0.8), width=3.0)
+        items.append((a, b, 1.0, 1.0, sigma, dict(kw, precision="fp32")))
+    for a, b, dt, ds, s, kw in items:
+        nbox = c.box_size(a.shape[0], a.shape[1], s)
+        c.set_config("fp32", nbox, "fp32-rowseg")
+    try:
+        got = c.track_batch(items)
+        for (a, b, dt, ds, s, kw), g in zip(items, got):
+            single = c.track(a, b, dt, ds, s, **kw)
+            assert c.last_path() == "fp32-rowseg"
+            for k in range(3):
+                assert np.array_equal(g[k], single[k]), (a.shape, s, k)
+            fkw = {k: v for k, v in kw.items() if k != "precision"}
+            ref = c.track(a, b, dt, ds, s, **fkw)
+            assert np.array_equal(single[2], ref[2])
+            rms, mx = _errs(ref, single)
+            assert rms <= 1e-4 and mx <= 1e-2, (a.shape, s, rms, mx)
+    finally:
+        for a, b, dt, ds, s, kw in items:
+            c.set_config("fp32", c.box_size(a.shape[0], a.shape[1], s), None)

@@ -59,6 +59,7 @@ CASES = [
      {"precision": "fp32", "thresh": 0.2, "skip": 3, "poff": 1, "interp": True, "kr": 0.5}),
     ("fp64 210^2 box 100 -b", 210, 24.0, {"bias": True}),
     ("fp64 96^2 box 24 -h", 96, 6.0, {"hires": True}),
+    ("fp32 512^2 box 64 -k (row-shared, FP32 row tasks)", 512, 15.0, {"precision": "fp32", "kr": 0.5}),
 ]
 
 
