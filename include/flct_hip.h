@@ -173,17 +173,21 @@ int flct_hip_last_refined(flct_hip_ctx *ctx, long long *refined);
  * the reference. */
 int flct_hip_last_kernel_ms(flct_hip_ctx *ctx, double *main_ms, double *refine_ms);
 
-/* Kernel path of the last track call: "fp64-fft", "fp32-fft",
- * "bf16-segment" or "bf16-pixel".  A precision form may run on any path
- * whose error is within the form's contract; FP64 serves every form.  No
- * counterpart in the reference. */
+/* Kernel path of the last track call: "fp64-fft" and "fp32-fft" (quad-FFT
+ * kernels, one pixel per 4-lane group), "fp64-rowseg" and "fp32-rowseg"
+ * (row-shared kernels: the box-row transforms of a row segment computed
+ * once, FP64 in both), "bf16-segment" and "bf16-pixel" (split-BF16 MFMA
+ * kernels) or "fp64-plane" (boxes above 96).  A precision form may run on
+ * any path whose error is within the form's contract; FP64 serves every
+ * form.  No counterpart in the reference. */
 const char *flct_hip_last_path(flct_hip_ctx *ctx);
 
 /* Runtime shape of the last call's main tracking kernel: *slots, the
  * pixels tracked together per block by the FFT kernels (fp64-fft, fp32-fft,
- * fp64-rowseg), and *segpx, the pixels per row segment of the row-segment
- * kernels (bf16-segment, fp64-rowseg); 0 stands for the compiled shape (its
- * slot count, 64 pixels per segment).  No counterpart in the reference. */
+ * fp64-rowseg, fp32-rowseg), and *segpx, the pixels per row segment of the
+ * row-segment kernels (bf16-segment, fp64-rowseg, fp32-rowseg); 0 stands
+ * for the compiled shape (its slot count, 64 pixels per segment).  No
+ * counterpart in the reference. */
 int flct_hip_last_shape(flct_hip_ctx *ctx, int *slots, int *segpx);
 
 /* Autotuner: runs every kernel path that meets the precision form of opts
@@ -195,8 +199,10 @@ int flct_hip_last_shape(flct_hip_ctx *ctx, int *slots, int *segpx);
  * runs the FP32 kernels for boxes up to 20 and above 64 and the row-segment
  * kernel from 24 to 64 (measured fastest there); its per-pixel MFMA kernels
  * are eligible up to box 72, above which their error can exceed the form's
- * contract.  sigma = 0 has one path: nothing to tune.  No counterpart in
- * the reference. */
+ * contract.  Untuned calls without a threshold, skip lattice, -b or -h run
+ * the row-shared kernel of their FP64 or FP32 precision on the box sizes
+ * where it was measured faster than the quad-FFT one.  sigma = 0 has one
+ * path: nothing to tune.  No counterpart in the reference. */
 int flct_hip_autotune(flct_hip_ctx *ctx, const double *d_f1, const double *d_f2, int nx, int ny,
                       const flct_hip_opts *opts);
 

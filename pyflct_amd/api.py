@@ -258,7 +258,8 @@ class Context:
 
     def last_path(self):
         """Kernel path of the last track call ("fp64-fft", "fp32-fft",
-        "bf16-segment", "bf16-pixel")."""
+        "fp64-rowseg", "fp32-rowseg", "bf16-segment", "bf16-pixel",
+        "fp64-plane")."""
         return _lib.lib().flct_hip_last_path(self._h).decode()
 
     def last_stats(self):
