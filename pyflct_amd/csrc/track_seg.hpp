@@ -97,7 +97,7 @@ struct SegCfg {
   static constexpr int RK = 4 * RW + 4;   // k row stride
   static constexpr int kTPlane = 32 * RK;
   static constexpr int kTEl = 4 * kTPlane;
-  static constexpr int kScratch = 128 * 8 + 64 * 8 + 9 * 8 + 8;  // zx, zp, nb, pad
+  static constexpr int kScratch = 128 * 8 + 64 * 8;  // zx, zp
   static constexpr int kGw = 128;  // window table gw[d + N - 1], 2N - 1 <= 127 doubles
   static constexpr int kLdsBytes =
       kTabEl * 2 + 256 * 8 + kTEl * 2 + 68 * 4 + kWaves * kScratch + kGw * 8;
@@ -162,7 +162,6 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
   unsigned char* wsc = reinterpret_cast<unsigned char*>(slist + 68) + wave * C::kScratch;
   float2* zx = reinterpret_cast<float2*>(wsc);  // [2][64] packed-column spectra
   float2* zp = zx + 128;                        // [64] packed cross power
-  double* nb = reinterpret_cast<double*>(zp + 64);
 
   // ---------------- tables to LDS, T planes zeroed (padding stays zero)
   {
@@ -672,84 +671,95 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           });
         });
       });
-      const float gmaxf = wave_max(lmax);
+      // the same in every lane: held as a wave-uniform (scalar) value, so the
+      // peak location and the fit below are scalar
+      const float gmaxf = __builtin_bit_cast(
+          float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, wave_max(lmax))));
       const float thr = gmaxf * (float)(1. - kTieTol);
       // cells within kTieTol of the maximum: exactly one (the maximum) settles
-      // the peak; with more the pixel is re-evaluated in FP64 and the location
-      // below (the first such cell of each slot, smallest scan index) only
-      // has to be deterministic.  A zero surface is the max == 0 case.
-      int nge = 0;
-      unsigned loc = 0xffffffffu;
-      if (gmaxf > 0.f) {
-        static_for<TL>([&](auto NT) {
-          static_for<2>([&](auto q) {
-            static_for<TL>([&](auto IT) {
-              static_for<2>([&](auto pm) {
-                static_for<4>([&](auto t) {
-                  const unsigned long long m = __ballot(fabsf(cv[NT][q][IT][pm][int(t)]) >= thr);
-                  // branch-free: a zero mask leaves nge and loc as they are
-                  nge += __popcll(m);
-                  // counted here: left to the compiler, the 64 counts sink to
-                  // the end and every mask stays live in scalar registers
-                  // (128 of them, spilled to vector lanes: 108 readlane and
-                  // writelane instructions per pixel)
-                  asm volatile("" : "+s"(nge));
-                  const int ln = __builtin_ctzll(m | (1ull << 63));
-                  const int ii = 16 * IT + (ln & 15);
-                  const int sx = (pm == 0) ? ii + Nh : ii;
-                  const int jj = 32 * NT + 8 * (ln >> 4) + 2 * t + q;
-                  const int sy = (jj < Nh) ? jj + Nh : jj - Nh;
-                  const unsigned L = m ? (unsigned)(sx * N + sy) : 0xffffffffu;
-                  loc = (L < loc) ? L : loc;
-                });
+      // the peak, and its register slot and lane are read off the sums below
+      // (for a single cell ssel is its slot and msel its lane bit); with more
+      // the pixel is re-evaluated in FP64, which rewrites all its outputs, so
+      // the cell these sums then name (masked into range) only has to be
+      // deterministic.  A zero surface is the max == 0 case.  Register slot
+      // s = (((NT 2 + q) TL + IT) 2 + pm) 4 + t, in the loop order below.
+      // held as vectors of at most 32 registers, which the compiler indexes
+      // in place for a wave-uniform slot (larger ones go through scratch)
+      constexpr int NS = TL * 2 * TL * 2 * 4;
+      constexpr int NH = (NS > 32) ? 32 : NS;
+      typedef float slotv __attribute__((ext_vector_type(NH)));
+      slotv cva[NS / NH];
+      static_for<TL>([&](auto NT) {
+        static_for<2>([&](auto q) {
+          static_for<TL>([&](auto IT) {
+            static_for<2>([&](auto pm) {
+              static_for<4>([&](auto t) {
+                constexpr int sl = (((NT * 2 + q) * TL + IT) * 2 + pm) * 4 + t;
+                cva[sl / NH][sl % NH] = cv[NT][q][IT][pm][int(t)];
               });
             });
           });
+        });
+      });
+      int nge = 0, ssel = 0;
+      unsigned long long msel = 0ull;
+      if (gmaxf > 0.f) {
+        static_for<NS>([&](auto sl) {
+          const unsigned long long m = __ballot(fabsf(cva[sl / NH][sl % NH]) >= thr);
+          // branch-free: a zero mask leaves the sums as they are
+          const int pc = __popcll(m);
+          nge += pc;
+          ssel += pc * int(sl);
+          msel |= m;
+          // summed here: left to the compiler, the sums sink to the end and
+          // every mask stays live in scalar registers
+          asm volatile("" : "+s"(nge), "+s"(ssel), "+s"(msel));
         });
       }
       const double gmax = (double)gmaxf;
       const bool nonzero = (gmaxf != 0.f);
       int ixmax = Nh, iymax = Nh;
       if (nonzero) {
-        ixmax = (int)(loc / (unsigned)N);
-        iymax = (int)(loc - (unsigned)ixmax * (unsigned)N);
+        // cell (NT, q, IT, pm, t) of lane (n, g): ii = 16 IT + n, x = ii + pm
+        // Nh, jj = 32 NT + 8 g + 2 t + q; in fftshift order (flct.c:1490-1510)
+        // it sits at row sx = (x + Nh) mod N, column sy = (jj + Nh) mod N
+        const int ln = __builtin_ctzll(msel | (1ull << 63));
+        const int sl = ssel & (NS - 1);
+        const int t = sl & 3, pm = (sl >> 2) & 1, IT = (sl >> 3) % TL;
+        const int q = (sl / (8 * TL)) & 1, NT = sl / (16 * TL);
+        const int ii = 16 * IT + (ln & 15);
+        ixmax = (pm == 0) ? ii + Nh : ii;
+        const int jj = 32 * NT + 8 * (ln >> 4) + 2 * t + q;
+        iymax = (jj < Nh) ? jj + Nh : jj - Nh;
       }
       const bool fit = nonzero && ixmax > 0 && ixmax < N - 1 && iymax > 0 && iymax < N - 1;
-      if (fit) {
-        // gather the 3x3 neighbourhood into LDS
-        int dxv[2][2];
-        bool xin[2][2];
-        static_for<TL>([&](auto IT) {
-          static_for<2>([&](auto pm) {
-            const int ii = 16 * IT + n;
-            const int sx = (pm == 0) ? ii + Nh : ii;
-            dxv[IT][pm] = sx - ixmax;
-            xin[IT][pm] = (ii < Nh) && (sx - ixmax >= -1) && (sx - ixmax <= 1);
-          });
-        });
-        static_for<TL>([&](auto NT) {
-          static_for<2>([&](auto q) {
-            static_for<4>([&](auto t) {
-              const int jj = 32 * NT + 8 * g + 2 * t + q;
-              const int sy = (jj < Nh) ? jj + Nh : jj - Nh;
-              const int dy = sy - iymax;
-              if ((jj < N) && (dy >= -1) && (dy <= 1)) {
-                static_for<TL>([&](auto IT) {
-                  static_for<2>([&](auto pm) {
-                    if (xin[IT][pm])
-                      nb[(dxv[IT][pm] + 1) * 3 + (dy + 1)] = (double)fabsf(cv[NT][q][IT][pm][int(t)]);
-                  });
-                });
-              }
-            });
-          });
-        });
-      }
-      wave_lds_fence();
+      // the 3x3 neighbourhood, read from its lanes (wave-uniform cells)
+      double nbv[9];
+      static_for<9>([&](auto c) {
+        nbv[int(c)] = 0.;
+        if (fit) {
+          const int sx = ixmax + int(c) / 3 - 1, sy = iymax + int(c) % 3 - 1;
+          const int pm = (sx >= Nh) ? 0 : 1;
+          const int ii = (sx >= Nh) ? sx - Nh : sx;
+          const int jj = (sy < Nh) ? sy + Nh : sy - Nh;
+          const int sl = ((((jj >> 5) * 2 + (jj & 1)) * TL + (ii >> 4)) * 2 + pm) * 4 + ((jj >> 1) & 3);
+          const int src = (ii & 15) + 16 * ((jj >> 3) & 3);
+          // a copy first: a bit cast of a vector element reads element 0
+          // with this compiler, whatever the index
+          float v = cva[0][sl % NH];
+          if constexpr (NS > NH) {
+            if (sl >= NH) v = cva[1][sl % NH];
+          }
+          const float r = __builtin_bit_cast(
+              float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+          nbv[int(c)] = (double)fabsf(r);
+        }
+      });
       bool flagged = nonzero && nge >= 2;
       double shiftsubx = 0., shiftsuby = 0.;
       if (fit) {
 #pragma clang fp contract(off)
+        const double* nb = nbv;
         const double c00 = nb[4];
         const double fx = 0.5 * (nb[7] - nb[1]);
         const double fy = 0.5 * (nb[5] - nb[3]);
@@ -763,7 +773,6 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         const double bound = kRelErr * gmax * hn * (1. + 4. * (fabs(shiftsubx) + fabs(shiftsuby)));
         if (!(bound <= kFitTol * fabs(det))) flagged = true;
       }
-      wave_lds_fence();  // nb is rewritten by the next pixel
       if (lane == 0) {
         const double shiftx = ((double)ixmax + shiftsubx) - (double)Nh;
         const double shifty = ((double)iymax + shiftsuby) - (double)Nh;
