@@ -425,6 +425,14 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
           cvl[X][sset] = u4_to_bf8(lv);
         });
       });
+      // Stage 2 runs at raised wave priority.  The two waves of a SIMD leave
+      // the segment barrier together and would otherwise keep step, their
+      // MFMA-dense stage 2 and VALU/SALU-dense tails coinciding; the wave in
+      // stage 2 now issues first and the pair drifts out of phase, so one
+      // wave's MFMAs overlap the other's VALU work (4096^2 box 64: 123.4 ->
+      // 114.0 ms per step, identical output; priority 3 the same, raising
+      // it also over the inverse stages less)
+      __builtin_amdgcn_s_setprio(1);
       static_for<TL>([&](auto LT) {
         // table operands are not held from one l-tile to the next
         asm volatile("" ::: "memory");
@@ -535,6 +543,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         });
       });
 
+      __builtin_amdgcn_s_setprio(0);
       // packed columns of both images, unpacked through the partner -l:
       // A0 = (z + conj z(-l)) / 2, A_nyq = (z - conj z(-l)) / (2i)
       wave_lds_fence();
