@@ -425,6 +425,7 @@ struct Plan {
   long long rs_grid = 0;  // k_track_rs: blocks
   int slots = 0;          // k_track / k_track_rs pixels per block (0: compiled count)
   int segpx = 0;          // k_track_seg / k_track_rs pixels per segment (0: 64)
+  long long seg_grid = 0; // k_track_seg: blocks at most (one scratch slice each)
   std::vector<double> gw, filt;
 };
 
@@ -514,6 +515,7 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Pla
     pl->big_grid = big_grid(c->num_cu, N, (long long)nx * ny);
   }
   pl->seg = (pl->path == PATH_BF16_SEG);
+  pl->seg_grid = pl->seg ? c->num_cu : 0;
   if (pl->r.skipon && o->interp) {
     int ixc = 0, iyc = 0;
     for (int i = 0; i < nx; i++)
@@ -545,7 +547,7 @@ struct Scratch {
   double* mean2 = nullptr;
   int* flags = nullptr;  // split-BF16 and FP32 pixels for FP64 re-evaluation
   int* elist = nullptr;  // edge-column pixels of the row-segment path
-  void* planes = nullptr;  // k_track_big: per block an N x N complex and an N x N real plane
+  void* planes = nullptr;  // per block: k_track_big N x N planes, k_track_rs spectra, k_track_seg V
 };
 
 constexpr int kNumScratch = 8;
@@ -562,6 +564,7 @@ void scratch_sizes(const Plan& pl, size_t sz[kNumScratch]) {
   sz[7] = (pl.path == PATH_FP64_PLANE)    ? (size_t)pl.big_grid * 24 * (size_t)N * N
           : (pl.path == PATH_FP64_ROWSEG || pl.path == PATH_FP32_ROWSEG)
               ? (size_t)pl.rs_grid * 16 * N * (size_t)(kRsSeg + N)
+          : pl.seg ? (size_t)pl.seg_grid * (size_t)kSegVBytes
                                           : 0;
 }
 
@@ -702,7 +705,8 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
         if ((rc = dispatch_track_bf16(N, lc, te))) return rc;
       }
       if ((rc = launch_track_seg(lc, ta, tb.segtab, o->thresh, pl.relative ? 1 : 0,
-                                 pl.r.skipon ? o->skip : 0, xoffset, yoffset, pl.segpx)))
+                                 pl.r.skipon ? o->skip : 0, xoffset, yoffset, pl.segpx,
+                                 (float*)sc.planes)))
         return rc;
     } else if ((rc = dispatch_track_bf16(N, lc, ta))) {
       return rc;

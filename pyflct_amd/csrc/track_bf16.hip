@@ -190,8 +190,10 @@ int launch_edge_list(const LaunchCtx& c, const int* list, const unsigned int* nl
   return hipGetLastError() == hipSuccess ? FLCT_HIP_OK : FLCT_HIP_E_HIP;
 }
 
+static_assert(SegCfg::kVFloats * 4LL == kSegVBytes, "k_track_seg scratch size");
+
 int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, double thresh,
-                     int thresh_rel, int skip, int xoffset, int yoffset, int segpx) {
+                     int thresh_rel, int skip, int xoffset, int yoffset, int segpx, float* vscr) {
   using C = SegCfg;
   const int N = 2 * a.nsize;
   SegArgs s;
@@ -202,6 +204,8 @@ int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, do
   s.skip = skip;
   s.xoffset = xoffset;
   s.yoffset = yoffset;
+  s.vscr = vscr;  // c.num_cu blocks of SegCfg::kVFloats
+  if (!vscr) return FLCT_HIP_E_ARG;
   s.jlo = a.nsize - 1;
   s.width = a.ny - N + 1;
   const int w = (segpx >= 8 && segpx < C::P) ? segpx : C::P;

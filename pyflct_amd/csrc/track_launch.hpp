@@ -78,7 +78,10 @@ void seg_tables(int N, double sigma, std::vector<unsigned short>& tab);
 int launch_edge_list(const LaunchCtx& c, const int* list, const unsigned int* nlist, int nx, int ny,
                      int nsize, int* out, unsigned int* count, long long* nedge);
 // segpx: pixels per row segment, 8..64 (0: 64)
+// k_track_seg scratch per block: the mean-correction transforms of one
+// segment, [64 pixels][2 l-tiles][2 images][2 column parities][re, im][16] floats
+constexpr long long kSegVBytes = 64LL * 2 * 2 * 2 * 2 * 16 * 4;
 int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, double thresh,
-                     int thresh_rel, int skip, int xoffset, int yoffset, int segpx);
+                     int thresh_rel, int skip, int xoffset, int yoffset, int segpx, float* vscr);
 
 }  // namespace flct

@@ -62,6 +62,7 @@ struct SegArgs {
   int nband;             // column bands of at most 64 pixels
   int nseg;              // nband * nx segments
   int xchunk;            // > 0: XCD-grouped order with this many row chunks (see k_track_seg)
+  float* vscr;           // per block SegCfg::kVFloats: the segment's mean-correction transforms
 };
 
 struct SegCfg {
@@ -99,6 +100,9 @@ struct SegCfg {
   static constexpr int kTEl = 4 * kTPlane;
   static constexpr int kScratch = 128 * 8 + 64 * 8;  // zx, zp
   static constexpr int kGw = 128;  // window table gw[d + N - 1], 2N - 1 <= 127 doubles
+  // mean-correction transforms of one segment (global, per block):
+  // [pixel P][l-tile 2][image 2][column parity 2][re, im][l 16] floats
+  static constexpr int kVFloats = P * 2 * 2 * 2 * 2 * 16;
   static constexpr int kLdsBytes =
       kTabEl * 2 + 256 * 8 + kTEl * 2 + 68 * 4 + kWaves * kScratch + kGw * 8;
   static_assert(kLdsBytes <= 160 * 1024, "k_track_seg needs more LDS than a CU has");
@@ -353,16 +357,52 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       });
     });
 
+    // ---------------- mean-correction transforms of the segment's pixels
+    // V_X(l; p) = sum_c' D_s(l, c') (cm(p + 2c' + s) - fbar_p), the stage-2
+    // transform of pixel p's real column vector over the even (s = 0) or odd
+    // (s = 1) columns, exact in real arithmetic beside the box-row spectra.
+    // One MFMA product per segment serves all its pixels: B column n is
+    // pixel 16 PT + n, so each output lane holds V for its column's pixel at
+    // the rows l = 16 LT + 4g + t that the pixel loop's accumulators take.
+    // The vector is formed in FP64 and rounded once (the column means and
+    // the box mean can be large beside their difference).
+    float* vb = A.vscr + (long long)blockIdx.x * C::kVFloats;
+    for (int cb = wave; cb < 16; cb += C::kWaves) {  // (X, s, PT)
+      const int n1 = fresh(n0), g1 = fresh(g0);
+      const SplitK sk = split_consts();
+      const int X = cb >> 3, sp = (cb >> 2) & 1, PT = cb & 3;
+      const int pp = 16 * PT + n1;
+      const bool pin = pp < P;
+      const double* mean = X ? a.mean2 : a.mean1;
+      const double fb = mean[(long long)imin * ns + (jb0 + (pin ? pp : 0) - (nsize - 1))];
+      const double* cr = cmd + 128 * X + pp + sp + 16 * g1;  // index <= 126
+      float v[8];
+      static_for<8>([&](auto e) { v[int(e)] = pin ? (float)(cr[2 * e] - fb) : 0.f; });
+      u32x4 hv, lv;
+      static_for<4>([&](auto d) {
+        unsigned hh, ll;
+        split2(v[2 * d], v[2 * d + 1], hh, ll, sk);
+        hv[int(d)] = hh;
+        lv[int(d)] = ll;
+      });
+      const bf16x8 vh = u4_to_bf8(hv), vl = u4_to_bf8(lv);
+      static_for<TL>([&](auto LT) {
+        const __bf16* D = TB + (n1 * RS + 8 * g1) + (sp ? C::kDo : C::kDe) + 16 * LT * RS;
+        const f32x4 rr = mfma3(ld8(D + 0 * PL), ld8(D + 1 * PL), vh, vl, z4);
+        const f32x4 ri = mfma3(ld8(D + 2 * PL), ld8(D + 3 * PL), vh, vl, z4);
+        float* o = vb + ((((pp * TL + LT) * 2 + X) * 2 + sp) * 2) * 16 + 4 * g1;
+        *reinterpret_cast<f32x4*>(o) = rr;
+        *reinterpret_cast<f32x4*>(o + 16) = ri;
+      });
+    }
+    __syncthreads();  // V of every pixel stored
+
     // ---------------- pixels of the segment, one per wave at a time
     for (int qi = wave; qi < cnt; qi += C::kWaves) {
       SegArgsPtr Ap = seg_args();
       const auto& a = Ap->t;
       const int p = __builtin_amdgcn_readfirstlane(slist[qi]);
       const int j = jb0 + p;
-      const int jmin = j - (nsize - 1);
-      // box means (the ones column already carries the image scale)
-      const double fbar[2] = {a.mean1[(long long)imin * ns + jmin],
-                              a.mean2[(long long)imin * ns + jmin]};
       // per-pixel copies of lane indices: keeps index math out of registers
       // across the pixel loop
       const int n = fresh(n0), g = fresh(g0);
@@ -403,28 +443,22 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       };
 
       CF4 Pp[2][2], Pm[2][2];  // [kt][LT] cross power
-      // split column vectors of the mean correction, v(c') = cm(p + 2c' +
-      // sset) - fbar_p, c' = 8g + e: the same for every l-tile, so they are
-      // formed once per pixel (2.4 % faster than per l-tile)
-      bf16x8 cvh[2][2], cvl[2][2];  // [X][sset]
-      static_for<2>([&](auto X) {
-        static_for<2>([&](auto sset) {
-          // formed in FP64 and rounded once: the column means and the box
-          // mean can be large beside their difference
-          const double* cr = cmd + 128 * X + p + sset + 16 * g;
-          float v[8];
-          static_for<8>([&](auto e) { v[int(e)] = (float)(cr[2 * e] - fbar[X]); });
-          u32x4 hv, lv;
-          static_for<4>([&](auto d) {
-            unsigned hh, ll;
-            split2(v[2 * d], v[2 * d + 1], hh, ll, sk);
-            hv[int(d)] = hh;
-            lv[int(d)] = ll;
+      // the pixel's mean-correction transforms for every l-tile, loaded at
+      // once (the same for every lane n)
+      f32x4 vld[TL][2][2][2];  // [LT][X][s][re, im]
+      {
+        const float* vp = vb + p * (TL * 128) + 4 * g;
+        static_for<TL>([&](auto LT) {
+          static_for<2>([&](auto X) {
+            static_for<2>([&](auto sp) {
+              static_for<2>([&](auto ri) {
+                vld[LT][X][sp][ri] = *reinterpret_cast<const f32x4*>(
+                    vp + (((LT * 2 + X) * 2 + sp) * 2 + ri) * 16);
+              });
+            });
           });
-          cvh[X][sset] = u4_to_bf8(hv);
-          cvl[X][sset] = u4_to_bf8(lv);
         });
-      });
+      }
       // Stage 2 runs at raised wave priority.  The two waves of a SIMD leave
       // the segment barrier together and would otherwise keep step, their
       // MFMA-dense stage 2 and VALU/SALU-dense tails coinciding; the wave in
@@ -436,23 +470,13 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       static_for<TL>([&](auto LT) {
         // table operands are not held from one l-tile to the next
         asm volatile("" ::: "memory");
-        // mean correction V(l) of both images: the stage-2 transform of the
-        // real column vector cm(p + c) - fbar (same for every k)
+        // mean correction V(l) of both images (same for every k)
         CF4 Ve[2], Vo[2];
         static_for<2>([&](auto X) {
-          f32x4 er, ei, orr, oi;
-          static_for<2>([&](auto sset) {
-            const bf16x8 vh = cvh[X][sset], vl = cvl[X][sset];
-            const DFr d = dtab(sset == 0 ? C::kDe : C::kDo, LT);
-            f32x4& rr = (sset == 0) ? er : orr;
-            f32x4& ri = (sset == 0) ? ei : oi;
-            rr = mfma3(d.rh, d.rl, vh, vl, z4);
-            ri = mfma3(d.ih, d.il, vh, vl, z4);
-          });
-          Ve[X].re = er;
-          Ve[X].im = ei;
-          Vo[X].re = orr;
-          Vo[X].im = oi;
+          Ve[X].re = vld[LT][X][0][0];
+          Ve[X].im = vld[LT][X][0][1];
+          Vo[X].re = vld[LT][X][1][0];
+          Vo[X].im = vld[LT][X][1][1];
         });
         // the stage-2 table fragments of this l-tile serve every k-tile and
         // both images (2.6 % faster than reloading them per image), and no
