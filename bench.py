@@ -60,10 +60,11 @@ MFMA_FLOPS = 2 * 16 * 16 * 32  # one v_mfma_f32_16x16x32_bf16
 
 def seg_mfma_flops(n):
     """MFMA flops the row-segment split-BF16 kernel executes per interior
-    pixel (boxes 36..64): 128 three-product groups per pixel (stage 2: 64,
-    mean correction: 16, inverse 1: 32, inverse 2: 16), plus the shared
-    box-row transform, 128 groups per 64-pixel segment."""
-    return (128 + 128 / 64) * 3 * MFMA_FLOPS
+    pixel (boxes 36..64): 112 three-product groups per pixel (stage 2: 64,
+    inverse 1: 32, inverse 2: 16), plus the two products shared by a 64-pixel
+    segment: the box-row transform (128 groups) and the mean-correction
+    transforms of its pixels (64 groups)."""
+    return (112 + (128 + 64) / 64) * 3 * MFMA_FLOPS
 
 
 def r2_mfma_flops(n, bias=False):

@@ -355,10 +355,12 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
   return v;
 }
 
-// Untuned choice: the split-BF16 request on boxes up to 20 runs the FP32
-// kernels (2048^2: 7.2 and 12.7 ms at boxes 16 and 20 against 14.5 and
-// 14.6 ms for the row-segment kernel, whose cost is flat below box 32).  On
-// boxes above 64 it runs the FP32 kernels (with their FP64 re-evaluation of
+// Untuned choice: the split-BF16 request on boxes up to 16 runs the FP32
+// kernels (2048^2 full-disk pair, whole-call ms, fp32-fft / fp32-rowseg /
+// bf16-segment: 8 3.0/2.0/11.4, 12 4.7/4.1/11.5, 16 7.3/5.9/11.6, 20
+// 12.5/18.1/11.7, 24 18.8/20.5/11.8, 32 30.5/25.4/12.0: the row-segment
+// kernel's cost is flat below box 32), the row-segment kernel from 20 (box
+// 20 with -b: FP32).  On boxes above 64 it runs the FP32 kernels (with their FP64 re-evaluation of
 // unsettled pixels), which are within the split-BF16 error bound and were
 // measured 29-41 % faster than the FP64 kernels and 2.9-3.7x faster than
 // the dense-DFT BF16 kernel there (2048^2, boxes 68..84).  Among the FP32
@@ -400,8 +402,9 @@ static int default_path(int precision, int N, int nx, int ny, int bias, int bf16
   if (precision == FLCT_HIP_FP32) return PATH_FP32;
   if (precision == FLCT_HIP_SPLIT_BF16) {
     if (bf16_kernel) return (N <= kBf16PixelMaxBox) ? PATH_BF16_PIXEL : PATH_FP32;
-    if (N > 64 || N <= 20) return PATH_FP32;
-    return seg_supported(N, nx, ny, bias) ? PATH_BF16_SEG : PATH_BF16_PIXEL;
+    if (N > 64 || N <= 16) return PATH_FP32;
+    if (seg_supported(N, nx, ny, bias)) return PATH_BF16_SEG;
+    return (N <= 20) ? PATH_FP32 : PATH_BF16_PIXEL;
   }
   return PATH_FP64;
 }

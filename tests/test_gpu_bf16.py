@@ -246,7 +246,7 @@ def test_paths_and_autotune():
 
 # every box size 4..96 in the FP32 and split-BF16 forms (their default kernel
 # paths: FP32 kernels for every box in FP32; in split-BF16 the FP32 kernels up
-# to box 20 and above 64, the row-segment kernel from 24 to 64)
+# to box 16 and above 64, the row-segment kernel from 20 to 64)
 BOXES = list(range(4, 100, 4))
 # boxes whose untuned FP32 kernel (FP32 form; split-BF16 outside 24..64) is
 # the row-shared one (fp32-rowseg) when no threshold is given
