@@ -67,11 +67,9 @@ struct SegArgs {
 
 struct SegCfg {
   static constexpr int P = 64;            // pixels per segment, at most
-#ifdef FLCT_SEG_WAVES
-  static constexpr int kWaves = FLCT_SEG_WAVES;  // waves per block (tuning builds)
-#else
+  // waves per block: one stage-1 column tile each (4- and 12-wave blocks
+  // measured slower)
   static constexpr int kWaves = 8;
-#endif
   static constexpr int kThreads = 64 * kWaves;
   // table row stride (bf16): 48 = 24 dwords.  A fragment load is a
   // ds_read_b128 of row n = lane & 15 at dword 4 (lane >> 4); measured on
@@ -225,6 +223,38 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
     const int imax0 = ((nx - 1) < (i + nsize)) ? nx - 1 : i + nsize;
     const int imin = (imax0 == nx - 1) ? nx - 1 - (2 * nsize - 1) : imin0;
 
+    // Stage-1 loads of this segment, issued before its barriers so that
+    // their latency overlaps the wait for the other waves (107.5 -> 105.7 ms
+    // per 4096^2 step): the box column values of both images of the wave's
+    // column tile ct = wave (rows x = 32 s + 8 g + e of this lane) and the
+    // stage-1 table fragments.
+    static_assert(C::kWaves == 8, "one stage-1 column tile per wave");
+    const int cb = jb0 - (nsize - 1);  // image column of T column 0
+    const int ucols = P + N - 1;       // T columns the boxes touch
+    // per-segment copies of the lane indices: keeps the table loads below
+    // from being hoisted out of the segment loop
+    const int n1 = fresh(n0), g1 = fresh(g0);
+    const int u = 16 * wave + n1;
+    const bool colok = u < ucols;
+    const bool ones = (u == 127);
+    const int ucl = colok ? u : 0;  // loads stay inside the image
+    double fva[2][16];
+    static_for<2>([&](auto X) {
+      const double* f = (X == 0) ? a.f1 : a.f2;
+      static_for<16>([&](auto q) {
+        const int x = 32 * (q >> 3) + 8 * g1 + (q & 7);
+        const int xr = (x < N) ? x : 0;
+        fva[X][int(q)] = f[(long long)(imin + xr) * ny + (cb + ucl)];
+      });
+    });
+    bf16x8 d1f[2][2][4];  // [kt][s][re_h, re_l, im_h, im_l]
+    static_for<TL>([&](auto kt) {
+      static_for<2>([&](auto s) {
+        const int off = (16 * kt + n1) * C::RS1 + 32 * s + 8 * g1;
+        static_for<4>([&](auto pl) { d1f[kt][s][pl] = ld8(A.tab + pl * C::kD1Plane + off); });
+      });
+    });
+
     __syncthreads();  // the previous segment's pixels are done with T and slist
     // tracked pixels of the segment: threshold and skip rules (flct.c:721-757)
     if (tid < 64) {
@@ -247,40 +277,10 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
     const int cnt = __builtin_amdgcn_readfirstlane(*scount);
     if (cnt == 0) continue;
 
-    const int cb = jb0 - (nsize - 1);  // image column of T column 0
-    const int ucols = P + N - 1;       // T columns the boxes touch
-
-    // ---------------- stage 1: column tile ct (T columns 16ct .. 16ct+15)
-    for (int ct = wave; ct < 8; ct += C::kWaves) {
-      // per-segment copies of the lane indices: keeps the table loads below
-      // from being hoisted out of the segment loop
-      const int n1 = fresh(n0), g1 = fresh(g0);
+    // ---------------- stage 1: column tile ct = wave (T columns 16ct .. 16ct+15)
+    {
+      const int ct = wave;
       const SplitK sk = split_consts();
-      const int u = 16 * ct + n1;
-      const bool colok = u < ucols;
-      const bool ones = (u == 127);
-      const int ucl = colok ? u : 0;  // loads stay inside the image
-      // rows x = 32 s + 8 g + e of this lane
-      // Every load of the tile is issued before any arithmetic: the box
-      // column values of both images, the window (from LDS) and the stage-1
-      // table fragments, which serve both images.
-      double fva[2][16];
-      static_for<2>([&](auto X) {
-        const double* f = (X == 0) ? a.f1 : a.f2;
-        static_for<16>([&](auto q) {
-          const int x = 32 * (q >> 3) + 8 * g1 + (q & 7);
-          const int xr = (x < N) ? x : 0;
-          fva[X][int(q)] = f[(long long)(imin + xr) * ny + (cb + ucl)];
-        });
-      });
-      const __bf16* D1 = A.tab;
-      bf16x8 d1f[2][2][4];  // [kt][s][re_h, re_l, im_h, im_l]
-      static_for<TL>([&](auto kt) {
-        static_for<2>([&](auto s) {
-          const int off = (16 * kt + n1) * C::RS1 + 32 * s + 8 * g1;
-          static_for<4>([&](auto pl) { d1f[kt][s][pl] = ld8(D1 + pl * C::kD1Plane + off); });
-        });
-      });
       float wr[16];
       static_for<16>([&](auto q) {
         const int x = 32 * (q >> 3) + 8 * g1 + (q & 7);
