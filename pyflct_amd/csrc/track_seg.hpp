@@ -255,6 +255,26 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       });
     });
 
+    // the pixel values of the threshold test (wave 0), read only with a
+    // threshold: with thresh = 0 the test fabsbar < thresh fails for every
+    // pixel (flct.c:721-722)
+    double f1t = 0., f2t = 0.;
+    if (tid < 64 && tid < P && A.thresh != 0.) {
+      const long long t = (long long)i * ny + (jb0 + tid);
+      f1t = a.f1[t];
+      f2t = a.f2[t];
+    }
+    // the box means of this wave's two columns of the mean-correction
+    // product below (images 0 and 1, pixel 16 (wave & 3) + n)
+    const int vpp = 16 * (wave & 3) + n1;
+    const bool vpin = vpp < P;
+    double fbv[2];
+    {
+      const long long vo = (long long)imin * ns + (jb0 + (vpin ? vpp : 0) - (nsize - 1));
+      fbv[0] = a.mean1[vo];
+      fbv[1] = a.mean2[vo];
+    }
+
     __syncthreads();  // the previous segment's pixels are done with T and slist
     // tracked pixels of the segment: threshold and skip rules (flct.c:721-757)
     if (tid < 64) {
@@ -262,8 +282,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       if (tid < P) {
 #pragma clang fp contract(off)
         const int j = jb0 + tid;
-        const long long t = (long long)i * ny + j;
-        const double fabsbar = 0.5 * (fabs(a.f1[t] + a.f2[t]));
+        const double fabsbar = 0.5 * (fabs(f1t + f2t));
         const bool below = (fabsbar < resolve_thresh(A.thresh, A.thresh_rel ? a.absmax : nullptr));
         bool noskip = true;
         if (A.skip) noskip = (!((i - A.xoffset) % A.skip)) && (!((j - A.yoffset) % A.skip));
@@ -367,14 +386,13 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
     // The vector is formed in FP64 and rounded once (the column means and
     // the box mean can be large beside their difference).
     float* vb = A.vscr + (long long)blockIdx.x * C::kVFloats;
-    for (int cb = wave; cb < 16; cb += C::kWaves) {  // (X, s, PT)
-      const int n1 = fresh(n0), g1 = fresh(g0);
+    static_for<2>([&](auto X) {  // (X, s, PT) = (X, (wave >> 2) & 1, wave & 3)
+      const int g1 = fresh(g0);
       const SplitK sk = split_consts();
-      const int X = cb >> 3, sp = (cb >> 2) & 1, PT = cb & 3;
-      const int pp = 16 * PT + n1;
-      const bool pin = pp < P;
-      const double* mean = X ? a.mean2 : a.mean1;
-      const double fb = mean[(long long)imin * ns + (jb0 + (pin ? pp : 0) - (nsize - 1))];
+      const int sp = (wave >> 2) & 1;
+      const int pp = vpp;
+      const bool pin = vpin;
+      const double fb = fbv[X];
       const double* cr = cmd + 128 * X + pp + sp + 16 * g1;  // index <= 126
       float v[8];
       static_for<8>([&](auto e) { v[int(e)] = pin ? (float)(cr[2 * e] - fb) : 0.f; });
@@ -394,7 +412,7 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         *reinterpret_cast<f32x4*>(o) = rr;
         *reinterpret_cast<f32x4*>(o + 16) = ri;
       });
-    }
+    });
     __syncthreads();  // V of every pixel stored
 
     // ---------------- pixels of the segment, one per wave at a time
