@@ -108,12 +108,13 @@ int flct_hip_track_device(flct_hip_ctx *ctx, const double *d_f1, const double *d
  * the reference: for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * Every table and scratch buffer of the call (box means and their row sums,
- * pixel lists, counters, transform tables, the planes of boxes above 96)
- * lives in one static 256 MiB device array that each device holds from
- * library load until the process ends; the caller passes only the inputs
- * and outputs.  A call whose scratch exceeds the array returns
- * FLCT_HIP_E_NOSCRATCH (every pixel tracked in FP64 fits up to about
- * 2900 x 2900 pixels); sigma <= 0 returns FLCT_HIP_E_UNSUPPORTED.  All
+ * pixel lists, counters, transform tables, the planes of boxes above 96,
+ * the row-segment kernels' per-block spectra) lives in one static 256 MiB
+ * device array that each device holds from library load until the process
+ * ends; the caller passes only the inputs and outputs.  A call whose
+ * scratch exceeds the array returns FLCT_HIP_E_NOSCRATCH (every pixel
+ * tracked fits up to about 2900 x 2900 pixels in FP64, 2600 x 2600 in
+ * split-BF16); sigma <= 0 returns FLCT_HIP_E_UNSUPPORTED.  All
  * calls on one device share the array: calls that overlap on one device,
  * from any context or thread, race.  Results equal flct_hip_track_device's
  * for the same inputs and options. */

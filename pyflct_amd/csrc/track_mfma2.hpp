@@ -422,6 +422,10 @@ __global__ void __launch_bounds__(R2Cfg<PIX>::kThreads, 2) k_track_r2(TrackArgs 
           });
         }
       };
+      // stages 1 and 2 at raised wave priority, so that the blocks sharing a
+      // CU drift out of phase and one's MFMAs overlap another's VALU work
+      // (as in k_track_seg)
+      __builtin_amdgcn_s_setprio(1);
       static_for<2>([&](auto X) {
         if constexpr (X == 0) {
           load_rows(X);
@@ -504,6 +508,7 @@ __global__ void __launch_bounds__(R2Cfg<PIX>::kThreads, 2) k_track_r2(TrackArgs 
         });
       });
 
+      __builtin_amdgcn_s_setprio(0);
       // ---------------- cross power
       CF4 Pp[2], Pm[2];
       static_for<2>([&](auto LT) {

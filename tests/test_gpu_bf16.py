@@ -87,6 +87,24 @@ def test_bf16_refinement_settles_outliers(ctx):
     assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
 
 
+@pytest.mark.parametrize("sigma,shift", [(15.0, 31), (15.0, -32), (9.0, 17)])
+def test_segment_peak_at_surface_edge(ctx, sigma, shift):
+    """Shifts of Nh - 1 and -Nh cells (box 64; Nh - 1 at box 36) put the
+    true peak on the border of the correlation surface, where no 3x3 fit runs
+    (flct.c:1536-1556) and the velocity is the peak cell itself: the cell the
+    row-segment kernel locates from its summed ballots must be FP64's."""
+    from conftest import smooth_pair
+    rng = np.random.default_rng(400 + shift)
+    a, _ = smooth_pair(rng, 200, 232, width=2.0)
+    b = np.roll(a, (shift, -shift), axis=(0, 1))
+    ref = ctx.track(a, b, 1.0, 1.0, sigma)
+    got = ctx.track(a, b, 1.0, 1.0, sigma, precision="split_bf16")
+    assert ctx.last_path() == "bf16-segment"
+    assert np.array_equal(got[2], ref[2])
+    rms, mx = _errs(ref, got)
+    assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
+
+
 def test_bf16_scale_invariance(ctx):
     """Inputs are scaled by powers of two before the split; results must not
     depend on the overall amplitude (tiny and huge images)."""
