@@ -22,7 +22,9 @@
 //            Wr^(k) V(l), V the same transform of the column vector
 //            cm(p + c) - fbar (exact in real arithmetic; every term is of the
 //            size of the box's own variation, and a constant box gives S = 0
-//            exactly, the reference's max == 0 case).
+//            exactly, the reference's max == 0 case).  V of all the segment's
+//            pixels is one MFMA product per segment (B column = pixel), kept
+//            in a block-private slice of device scratch.
 //   cross power P = conj(S1) S2 (flct.c:1461-1477; normfac dropped: the
 //            argmax and the fit are scale invariant), times filter^2
 //            (flct.c:1445-1457); the packed column is unpacked through l <-> -l.
@@ -39,7 +41,8 @@
 // lane's column k, which is exactly the A operand (row k, 8 contraction
 // values l) of inverse 1; inverse 1's outputs are rows k for the lane's column
 // jj, the A operand of inverse 2.  So after stage 1 nothing of a pixel passes
-// through LDS except the packed-column exchange and the 3x3 neighbourhood.
+// through LDS except the packed-column exchange; the 3x3 neighbourhood of the
+// peak is read from its lanes with v_readlane.
 // Products are hi*hi + hi*lo + lo*hi of BF16 splits accumulated in FP32
 // (v_mfma_f32_16x16x32_bf16); the argmax uses the values only, the 3x3 Taylor
 // fit (flct.c:1536-1556) is FP64.  Pixels the BF16 surface cannot settle (a
