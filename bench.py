@@ -92,14 +92,27 @@ def host_cpu():
     FFTW3 library is installed (ldconfig -p; ROCm's GPU libhipfftw does not
     count)."""
     model = "unknown"
+    fields = {}
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                if ":" not in line:
+                    if fields:
+                        break  # the first processor's block
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                fields.setdefault(k, v)
     except OSError:
         pass
+    if "model name" in fields:
+        model = fields["model name"]
+        # a virtualised host may report only the brand: add the family,
+        # model and stepping numbers, which identify the core generation
+        extra = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields]
+        if "hypervisor" in fields.get("flags", "").split():
+            extra.append("virtual machine")
+        if extra:
+            model = f"{model} ({', '.join(extra)})"
     try:
         avail = len(os.sched_getaffinity(0))
     except AttributeError:
