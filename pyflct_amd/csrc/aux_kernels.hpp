@@ -87,6 +87,16 @@ __global__ void __launch_bounds__(256) k_boxmean(const double* __restrict__ hs, 
     if (r0 + k < nr) mean[(long long)(r0 + k) * ns + sidx] = acc[k] / ((double)N * N);
 }
 
+// Transposed column strip: out[t * nx + i] = f[i * ny + c0 + t] for the N
+// columns c0 .. c0 + N - 1 of an nx x ny image (writes coalesced along i).
+__global__ void __launch_bounds__(256) k_strip(const double* __restrict__ f, int nx, int ny, int c0,
+                                               int N, double* __restrict__ out) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)N * nx) return;
+  const int t = (int)(e / nx), i = (int)(e - (long long)t * nx);
+  out[e] = f[(long long)i * ny + c0 + t];
+}
+
 struct SelectArgs {
   const double* f1;
   const double* f2;

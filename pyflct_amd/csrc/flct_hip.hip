@@ -429,6 +429,7 @@ struct Plan {
   int slots = 0;          // k_track / k_track_rs pixels per block (0: compiled count)
   int segpx = 0;          // k_track_seg / k_track_rs pixels per segment (0: 64)
   long long seg_grid = 0; // k_track_seg: blocks at most (one scratch slice each)
+  bool strip = false;     // k_track_seg also runs the transposed edge-column strips
   std::vector<double> gw, filt;
 };
 
@@ -519,6 +520,10 @@ int make_plan(int nx, int ny, const flct_hip_opts* o, const flct_hip_ctx* c, Pla
   }
   pl->seg = (pl->path == PATH_BF16_SEG);
   pl->seg_grid = pl->seg ? c->num_cu : 0;
+  // the clamped edge columns outside the clamped edge rows run through the
+  // row-segment kernel on transposed column strips; the four corners stay
+  // on the per-pixel kernel
+  pl->strip = pl->seg && nx >= N;
   if (pl->r.skipon && o->interp) {
     int ixc = 0, iyc = 0;
     for (int i = 0; i < nx; i++)
@@ -555,6 +560,16 @@ struct Scratch {
 
 constexpr int kNumScratch = 8;
 
+// The transposed edge strips of the split-BF16 form (after the row-segment
+// kernel's slices in the planes buffer): per side (left, right) and image the
+// strip (N x nx), its row sums (N x (nx - N + 1)) and its box means
+// (nx - N + 1), as doubles.
+size_t strip_bytes(const Plan& pl) {
+  if (!pl.strip) return 0;
+  const size_t nss = (size_t)(pl.nx - pl.N + 1);
+  return 4 * ((size_t)pl.N * pl.nx + (size_t)pl.N * nss + nss) * sizeof(double);
+}
+
 // Scratch bytes of one plan: list, hs1, hs2, mean1, mean2, flags, elist, planes
 void scratch_sizes(const Plan& pl, size_t sz[kNumScratch]) {
   const long long npix = (long long)pl.nx * pl.ny;
@@ -567,7 +582,7 @@ void scratch_sizes(const Plan& pl, size_t sz[kNumScratch]) {
   sz[7] = (pl.path == PATH_FP64_PLANE)    ? (size_t)pl.big_grid * 24 * (size_t)N * N
           : (pl.path == PATH_FP64_ROWSEG || pl.path == PATH_FP32_ROWSEG)
               ? (size_t)pl.rs_grid * 16 * N * (size_t)(kRsSeg + N)
-          : pl.seg ? (size_t)pl.seg_grid * (size_t)kSegVBytes
+          : pl.seg ? (size_t)pl.seg_grid * (size_t)kSegVBytes + strip_bytes(pl)
                                           : 0;
 }
 
@@ -697,7 +712,7 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
     if (pl.seg) {
       long long nedge = 0;
       if ((rc = launch_edge_list(lc, ta.list, ta.nwork_dev, nx, ny, ta.nsize, sc.elist, counts + 2,
-                                 &nedge)))
+                                 &nedge, pl.strip)))
         return rc;
       if (nedge != 0) {
         TrackArgs te = ta;
@@ -711,6 +726,48 @@ int enqueue_track(const LaunchCtx& lc, const Plan& pl, const Tables& tb, const S
                                  pl.r.skipon ? o->skip : 0, xoffset, yoffset, pl.segpx,
                                  (float*)sc.planes)))
         return rc;
+      if (pl.strip) {
+        // Edge columns: the left and right N-column strips of both images,
+        // transposed, with their box means; in a strip the image's clamped
+        // edge columns are rows whose pixels (the image rows whose boxes are
+        // not clamped) are interior columns, which the row-segment kernel
+        // tracks with the same windows and boxes (flct.c:773-870)
+        const int nss = nx - N + 1;
+        const size_t simg = (size_t)N * nx, shs = (size_t)N * nss;
+        double* sb = (double*)((char*)sc.planes + (size_t)pl.seg_grid * (size_t)kSegVBytes);
+        const long long nbx_s = (nss + kMeanTile - 1) / kMeanTile, nby_s = (nss + 255) / 256;
+        for (int side = 0; side < 2; ++side) {
+          const int c0 = side ? ny - N : 0;
+          double* img[2];
+          double* mn[2];
+          for (int X = 0; X < 2; ++X) {
+            const int k = 2 * side + X;
+            img[X] = sb + k * simg;
+            double* hs = sb + 4 * simg + k * shs;
+            mn[X] = sb + 4 * simg + 4 * shs + (size_t)k * nss;
+            hipLaunchKernelGGL(k_strip, dim3((unsigned)((simg + 255) / 256)), dim3(256), 0, s,
+                               X ? f2 : f1, nx, ny, c0, N, img[X]);
+            hipLaunchKernelGGL(k_rowsum, dim3((unsigned)(N * nbx_s)), dim3(kMeanTile), rs_lds, s,
+                               (const double*)img[X], N, nx, N, hs);
+            hipLaunchKernelGGL(k_boxmean, dim3((unsigned)nby_s), dim3(256), 0, s, (const double*)hs,
+                               N, nx, N, mn[X]);
+          }
+          HIPCHK(hipGetLastError());
+          TrackArgs ts = ta;
+          ts.f1 = img[0];
+          ts.f2 = img[1];
+          ts.nx = N;
+          ts.ny = nx;
+          ts.mean1 = mn[0];
+          ts.mean2 = mn[1];
+          ts.list = nullptr;
+          const SegStrip st{side ? ta.nsize : 0, side ? ta.nsize : ta.nsize - 1, c0, ny};
+          if ((rc = launch_track_seg(lc, ts, tb.segtab, o->thresh, pl.relative ? 1 : 0,
+                                     pl.r.skipon ? o->skip : 0, xoffset, yoffset, pl.segpx,
+                                     (float*)sc.planes, &st)))
+            return rc;
+        }
+      }
     } else if ((rc = dispatch_track_bf16(N, lc, ta))) {
       return rc;
     }

@@ -172,19 +172,22 @@ void seg_tables(int N, double sigma, std::vector<unsigned short>& tab) {
 }
 
 int launch_edge_list(const LaunchCtx& c, const int* list, const unsigned int* nlist, int nx, int ny,
-                     int nsize, int* out, unsigned int* count, long long* nedge) {
+                     int nsize, int* out, unsigned int* count, long long* nedge, bool corners_only) {
   const int jlo = nsize - 1, jhi = ny - nsize - 1;
+  // rows whose boxes are not clamped: their edge pixels run through the strips
+  const int ilo = corners_only ? nsize - 1 : 1, ihi = corners_only ? nx - nsize - 1 : 0;
+  const int nskip = (ihi >= ilo) ? ihi - ilo + 1 : 0;
   if (list) {
     // count must be zero on entry; the per-pixel kernel reads it on the device
     hipLaunchKernelGGL(k_edge_list, dim3(1024), dim3(256), 0, c.stream, list, nlist, nx, ny, jlo, jhi,
-                       out, count);
+                       ilo, ihi, out, count);
     *nedge = -1;
   } else {
-    const long long ne = (long long)nx * (jlo + (ny - 1 - jhi));
+    const long long ne = (long long)(nx - nskip) * (jlo + (ny - 1 - jhi));
     if (ne > 0)
       hipLaunchKernelGGL(k_edge_list, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c.stream,
-                         (const int*)nullptr, (const unsigned int*)nullptr, nx, ny, jlo, jhi, out,
-                         count);
+                         (const int*)nullptr, (const unsigned int*)nullptr, nx, ny, jlo, jhi, ilo, ihi,
+                         out, count);
     *nedge = ne;
   }
   return hipGetLastError() == hipSuccess ? FLCT_HIP_OK : FLCT_HIP_E_HIP;
@@ -193,7 +196,8 @@ int launch_edge_list(const LaunchCtx& c, const int* list, const unsigned int* nl
 static_assert(SegCfg::kVFloats * 4LL == kSegVBytes, "k_track_seg scratch size");
 
 int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, double thresh,
-                     int thresh_rel, int skip, int xoffset, int yoffset, int segpx, float* vscr) {
+                     int thresh_rel, int skip, int xoffset, int yoffset, int segpx, float* vscr,
+                     const SegStrip* st) {
   using C = SegCfg;
   const int N = 2 * a.nsize;
   SegArgs s;
@@ -206,11 +210,17 @@ int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, do
   s.yoffset = yoffset;
   s.vscr = vscr;  // c.num_cu blocks of SegCfg::kVFloats
   if (!vscr) return FLCT_HIP_E_ARG;
+  s.strip = st ? 1 : 0;
+  s.rlo = st ? st->rlo : 0;
+  s.nrows = st ? st->nrows : a.nx;
+  s.cb0 = st ? st->cb0 : 0;
+  s.nyo = st ? st->nyo : a.ny;
+  if (s.nrows < 1) return FLCT_HIP_OK;
   s.jlo = a.nsize - 1;
   s.width = a.ny - N + 1;
   const int w = (segpx >= 8 && segpx < C::P) ? segpx : C::P;
   s.nband = (s.width + w - 1) / w;
-  const long long nseg = (long long)s.nband * a.nx;
+  const long long nseg = (long long)s.nband * s.nrows;
   if (nseg > 0x7fffffffLL) return FLCT_HIP_E_UNSUPPORTED;
   s.nseg = (int)nseg;
   const size_t lds = C::kLdsBytes;
@@ -223,7 +233,7 @@ int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, do
   // XCD order when the bands split evenly over the 8 XCDs and each XCD's
   // blocks evenly over its bands (k_track_seg)
   s.xchunk = 0;
-  if (grid % 8 == 0 && s.nband % 8 == 0 && (grid / 8) % (s.nband / 8) == 0)
+  if (!st && grid % 8 == 0 && s.nband % 8 == 0 && (grid / 8) % (s.nband / 8) == 0)
     s.xchunk = (int)((grid / 8) / (s.nband / 8));
   if (N <= 32)
     hipLaunchKernelGGL(k_track_seg<1>, dim3((unsigned)grid), dim3(C::kThreads), lds, c.stream, s);

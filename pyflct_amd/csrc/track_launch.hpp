@@ -72,16 +72,24 @@ int dispatch_track_bf16(int N, const LaunchCtx& c, const TrackArgs& a);
 // launch_edge_list lists the tracked pixels of the clamped edge columns: from
 // a tracked-pixel list of device length *nlist into out / *count (zero on
 // entry; *nedge = -1, the count stays on the device), or every edge pixel
-// when list is null (*nedge = their number).
+// when list is null (*nedge = their number).  corners_only: only those in
+// clamped edge rows too (the others run through the transposed strips).
 bool seg_supported(int N, int nx, int ny, int bias);
 void seg_tables(int N, double sigma, std::vector<unsigned short>& tab);
 int launch_edge_list(const LaunchCtx& c, const int* list, const unsigned int* nlist, int nx, int ny,
-                     int nsize, int* out, unsigned int* count, long long* nedge);
+                     int nsize, int* out, unsigned int* count, long long* nedge, bool corners_only);
 // segpx: pixels per row segment, 8..64 (0: 64)
 // k_track_seg scratch per block: the mean-correction transforms of one
 // segment, [64 pixels][2 l-tiles][2 images][2 column parities][re, im][16] floats
 constexpr long long kSegVBytes = 64LL * 2 * 2 * 2 * 2 * 16 * 4;
+// Strip mode of k_track_seg (SegArgs::strip): rows [rlo, rlo + nrows) of a
+// transposed edge strip whose row t is image column cb0 + t; nyo is the
+// image's ny.
+struct SegStrip {
+  int rlo, nrows, cb0, nyo;
+};
 int launch_track_seg(const LaunchCtx& c, const TrackArgs& a, const void* tab, double thresh,
-                     int thresh_rel, int skip, int xoffset, int yoffset, int segpx, float* vscr);
+                     int thresh_rel, int skip, int xoffset, int yoffset, int segpx, float* vscr,
+                     const SegStrip* st = nullptr);
 
 }  // namespace flct

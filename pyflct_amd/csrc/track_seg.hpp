@@ -66,6 +66,15 @@ struct SegArgs {
   int nseg;              // nband * nx segments
   int xchunk;            // > 0: XCD-grouped order with this many row chunks (see k_track_seg)
   float* vscr;           // per block SegCfg::kVFloats: the segment's mean-correction transforms
+  // rows run: [rlo, rlo + nrows) (the whole image: 0, nx)
+  int rlo, nrows;
+  // strip mode: the images are a transposed edge strip, strip row t = image
+  // column cb0 + t and strip column = image row, so that the strip's
+  // interior columns are the image's clamped edge columns (flct.c:780-786)
+  // outside its edge rows.  Outputs go to image pixel (strip column) * nyo +
+  // cb0 + (strip row), with vx and vy exchanged; the skip lattice is tested
+  // in image coordinates.
+  int strip, cb0, nyo;
 };
 
 struct SegCfg {
@@ -216,8 +225,8 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         if (m > 0. && m < 1e300) sc[X] = ldexp(1., -ilogb(m));
       });
     }
-    const int band = (A.xchunk > 0) ? sband : sidx / nx;
-    const int i = (A.xchunk > 0) ? sidx : sidx - band * nx;
+    const int band = (A.xchunk > 0) ? sband : sidx / A.nrows;
+    const int i = (A.xchunk > 0) ? sidx : A.rlo + (sidx - band * A.nrows);
     const int jb0 = A.jlo + (int)((long long)band * A.width / A.nband);
     const int jb1 = A.jlo + (int)((long long)(band + 1) * A.width / A.nband);
     const int P = jb1 - jb0;
@@ -288,7 +297,9 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         const double fabsbar = 0.5 * (fabs(f1t + f2t));
         const bool below = (fabsbar < resolve_thresh(A.thresh, A.thresh_rel ? a.absmax : nullptr));
         bool noskip = true;
-        if (A.skip) noskip = (!((i - A.xoffset) % A.skip)) && (!((j - A.yoffset) % A.skip));
+        // image coordinates of the pixel (transposed in strip mode)
+        const int io = A.strip ? j : i, jo = A.strip ? A.cb0 + i : j;
+        if (A.skip) noskip = (!((io - A.xoffset) % A.skip)) && (!((jo - A.yoffset) % A.skip));
         tr = (!below) && noskip;
       }
       const unsigned long long m = __ballot(tr);
@@ -823,9 +834,12 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       if (lane == 0) {
         const double shiftx = ((double)ixmax + shiftsubx) - (double)Nh;
         const double shifty = ((double)iymax + shiftsuby) - (double)Nh;
-        const long long pix = (long long)i * ny + j;
-        a.vx[pix] = shifty * a.deltinv * a.deltas;
-        a.vy[pix] = shiftx * a.deltinv * a.deltas;
+        const long long pix =
+            A.strip ? (long long)j * A.nyo + (A.cb0 + i) : (long long)i * ny + j;
+        // in strip mode the surface's rows are image columns: shiftx is the
+        // image's column shift
+        a.vx[pix] = (A.strip ? shiftx : shifty) * a.deltinv * a.deltas;
+        a.vy[pix] = (A.strip ? shifty : shiftx) * a.deltinv * a.deltas;
         a.vm[pix] = 1.;
         if (flagged && a.flag_list) a.flag_list[atomicAdd(a.flag_count, 1u)] = (int)pix;
       }
@@ -838,9 +852,17 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
 // tracked-pixel list (threshold or skip; its length in *nlist) the list is
 // filtered into out / *count by a grid-stride loop; otherwise the edge
 // columns of every row are enumerated.
+// Edge pixels for the per-pixel kernel: tracked pixels whose box is clamped
+// at the left or right image edge (columns outside [jlo, jhi]) and whose row
+// is outside [ilo, ihi] (rows inside it run through the transposed edge
+// strips; an empty range, ilo > ihi, keeps every row).  With a tracked-pixel
+// list (threshold or skip; its length in *nlist) the list is filtered into
+// out / *count by a grid-stride loop; otherwise the edge pixels are
+// enumerated.
 __global__ void k_edge_list(const int* __restrict__ list, const unsigned int* __restrict__ nlist,
-                            int nx, int ny, int jlo, int jhi, int* __restrict__ out,
+                            int nx, int ny, int jlo, int jhi, int ilo, int ihi, int* __restrict__ out,
                             unsigned int* __restrict__ count) {
+  const int nskip = (ihi >= ilo) ? ihi - ilo + 1 : 0;  // rows left to the strips
   if (list) {
     const long long n = (long long)*nlist;
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -850,8 +872,9 @@ __global__ void k_edge_list(const int* __restrict__ list, const unsigned int* __
       int p = 0;
       if (t < n) {
         p = list[t];
-        const int j = p % ny;
-        e = (j < jlo) || (j > jhi);
+        const int r = p / ny;
+        const int j = p - r * ny;
+        e = ((j < jlo) || (j > jhi)) && !(nskip > 0 && r >= ilo && r <= ihi);
       }
       const unsigned long long mask = __ballot(e);
       const int ln = threadIdx.x & 63;
@@ -863,8 +886,9 @@ __global__ void k_edge_list(const int* __restrict__ list, const unsigned int* __
   } else {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int we = jlo + (ny - 1 - jhi);
-    if (t < (long long)nx * we) {
-      const int r = (int)(t / we), c = (int)(t - (long long)r * we);
+    if (t < (long long)(nx - nskip) * we) {
+      const int rr = (int)(t / we), c = (int)(t - (long long)rr * we);
+      const int r = (nskip > 0 && rr >= ilo) ? rr + nskip : rr;
       const int j = (c < jlo) ? c : jhi + 1 + (c - jlo);
       out[t] = r * ny + j;
     }
