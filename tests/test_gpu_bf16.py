@@ -105,6 +105,37 @@ def test_segment_peak_at_surface_edge(ctx, sigma, shift):
     assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
 
 
+@pytest.mark.parametrize("nx,ny,sigma,kw", [
+    (150, 190, 15.0, {}),                                  # box 64, clamped rows and columns
+    (64, 100, 15.0, {}),                                   # nx = N: one unclamped row
+    (97, 120, 9.0, {"thresh": 0.3}),                       # box 36, relative threshold
+    (180, 170, 7.0, {"skip": 3, "poff": 1, "qoff": 2}),    # box 28 (single tile), skip lattice
+])
+def test_segment_edge_columns_through_strips(ctx, nx, ny, sigma, kw):
+    """The clamped edge columns of the split-BF16 form run through the
+    row-segment kernel on transposed column strips (their boxes and windows,
+    flct.c:773-870, are those of interior columns there), the four corners
+    through the per-pixel kernel.  Against FP64 on the edge columns alone:
+    the mask bit-exact, the velocities within the form's bounds."""
+    from conftest import smooth_pair
+    rng = np.random.default_rng(nx * 1000 + ny)
+    a, b = smooth_pair(rng, nx, ny, shift=(0.7, -0.4), width=2.0)
+    ref = ctx.track(a, b, 1.0, 1.0, sigma, **kw)
+    got = ctx.track(a, b, 1.0, 1.0, sigma, precision="split_bf16", **kw)
+    assert ctx.last_path() == "bf16-segment"
+    assert np.array_equal(got[2], ref[2])
+    nbox = ctx.box_size(nx, ny, sigma)
+    edge = np.zeros((nx, ny), bool)
+    edge[:, :nbox // 2 - 1] = True
+    edge[:, ny - nbox // 2:] = True
+    tracked = edge & (ref[2] == 1.0)
+    assert tracked.sum() > 0
+    e = [ref[k][tracked] - got[k][tracked] for k in (0, 1)]
+    rms = float(np.sqrt(np.mean(e[0] ** 2 + e[1] ** 2)))
+    mx = float(max(np.max(np.abs(e[0])), np.max(np.abs(e[1]))))
+    assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
+
+
 def test_bf16_scale_invariance(ctx):
     """Inputs are scaled by powers of two before the split; results must not
     depend on the overall amplitude (tiny and huge images)."""
