@@ -120,14 +120,13 @@ struct SegCfg {
 
 __device__ __forceinline__ bf16x8 u4_to_bf8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-// 8 consecutive bf16 of an LDS row starting at element e (any alignment);
-// odd is e & 1, uniform over the wave.  Branch-free: five dwords from the
-// one at or below element e, shifted by 2 bytes when e is odd (by 0, the
-// low dword itself, when it is even), so the loads of one tile can be
-// scheduled across the MFMAs of the previous one.
-__device__ __forceinline__ bf16x8 ld8_any(const __bf16* row, int e, bool odd) {
-  const unsigned sh = odd ? 2u : 0u;
-  const unsigned* q = reinterpret_cast<const unsigned*>(row + e - (odd ? 1 : 0));
+// 8 consecutive bf16 of an LDS row from any element (odd starts uniform over
+// the wave).  Branch-free: five dwords from the one at or below the start,
+// q, shifted by sh = 2 bytes when the start is odd (by 0, the low dword
+// itself, when it is even), so the loads of one tile can be scheduled
+// across the MFMAs of the previous one.  Callers form q from one address
+// per run, so that every read is an immediate offset from it.
+__device__ __forceinline__ bf16x8 ld8_sh(const unsigned* q, unsigned sh) {
   const unsigned d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
   u32x4 r;
   r[0] = __builtin_amdgcn_alignbyte(d1, d0, sh);
@@ -174,8 +173,12 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
   double* gwl = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(slist + 68) +
                                           C::kWaves * C::kScratch);  // window table
   unsigned char* wsc = reinterpret_cast<unsigned char*>(slist + 68) + wave * C::kScratch;
-  float2* zx = reinterpret_cast<float2*>(wsc);  // [2][64] packed-column spectra
-  float2* zp = zx + 128;                        // [64] packed cross power
+  // Packed DC / Nyquist column per wave, planar: zx [image][re, im][64] the
+  // spectra, zp [re, im][64] their cross power.  Row l < Nh sits at l and
+  // row l + Nh at 32 + l, so a group of four rows is one aligned f32x4 for
+  // every box size.
+  float* zx = reinterpret_cast<float*>(wsc);
+  float* zp = zx + 256;
 
   // ---------------- tables to LDS, T planes zeroed (padding stays zero)
   {
@@ -509,12 +512,21 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         const int k = 16 * kt + n;
         CF4 s1[TL][2];  // image 0's S(l), S(l + Nh) of every l-tile
         static_for<2>([&](auto X) {
-          const __bf16* tre = TTl + (X * 2 + pe) * TP + 16 * kt * RK + be;
-          const __bf16* tro = TTl + (X * 2 + po) * TP + 16 * kt * RK + bo;
-          const bf16x8 eh = ld8_any(tre + 0 * RW, 0, oe), el = ld8_any(tre + 1 * RW, 0, oe);
-          const bf16x8 euh = ld8_any(tre + 2 * RW, 0, oe), eul = ld8_any(tre + 3 * RW, 0, oe);
-          const bf16x8 oh = ld8_any(tro + 0 * RW, 0, oo), ol = ld8_any(tro + 1 * RW, 0, oo);
-          const bf16x8 ouh = ld8_any(tro + 2 * RW, 0, oo), oul = ld8_any(tro + 3 * RW, 0, oo);
+          // the even and odd column runs of this k-tile and image, each from
+          // one address (the element offset, less one where it is odd, held
+          // opaque in a scalar register): the 20 dword reads of a run are
+          // immediate offsets from it, where a constant folded into the
+          // address took an add per read pair beyond the 1 KB reach of the
+          // paired-read offsets
+          const int ofe = fresh_s((X * 2 + pe) * TP + 16 * kt * RK + be - (oe ? 1 : 0));
+          const int ofo = fresh_s((X * 2 + po) * TP + 16 * kt * RK + bo - (oo ? 1 : 0));
+          const unsigned* tre = reinterpret_cast<const unsigned*>(TTl + ofe);
+          const unsigned* tro = reinterpret_cast<const unsigned*>(TTl + ofo);
+          const unsigned she = oe ? 2u : 0u, sho = oo ? 2u : 0u;
+          const bf16x8 eh = ld8_sh(tre + 0 * RW / 2, she), el = ld8_sh(tre + 1 * RW / 2, she);
+          const bf16x8 euh = ld8_sh(tre + 2 * RW / 2, she), eul = ld8_sh(tre + 3 * RW / 2, she);
+          const bf16x8 oh = ld8_sh(tro + 0 * RW / 2, sho), ol = ld8_sh(tro + 1 * RW / 2, sho);
+          const bf16x8 ouh = ld8_sh(tro + 2 * RW / 2, sho), oul = ld8_sh(tro + 3 * RW / 2, sho);
           static_for<TL>([&](auto LT) {
             // the table fragments of this l-tile, reloaded per use rather
             // than held across the loops
@@ -544,15 +556,28 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
               sm.im[int(t)] = ei[int(t)] - oi[int(t)];
             });
             if constexpr (kt == 0) {
-              // the packed DC / Nyquist column to the exchange buffer
+              // the packed DC / Nyquist column to the exchange buffer: a
+              // group of four rows below Nh as one vector store per part, a
+              // group across Nh row by row
               if (n == 0) {
-                static_for<4>([&](auto t) {
-                  const int l = 16 * LT + 4 * g + t;
-                  if (l < Nh) {
-                    zx[64 * X + l] = make_float2(sp.re[int(t)], sp.im[int(t)]);
-                    zx[64 * X + l + Nh] = make_float2(sm.re[int(t)], sm.im[int(t)]);
-                  }
-                });
+                const int l0 = 16 * LT + 4 * g;
+                float* zr = zx + 128 * X;
+                if (l0 + 3 < Nh) {
+                  *reinterpret_cast<f32x4*>(zr + l0) = sp.re;
+                  *reinterpret_cast<f32x4*>(zr + 64 + l0) = sp.im;
+                  *reinterpret_cast<f32x4*>(zr + 32 + l0) = sm.re;
+                  *reinterpret_cast<f32x4*>(zr + 96 + l0) = sm.im;
+                } else {
+                  static_for<4>([&](auto t) {
+                    const int l = l0 + t;
+                    if (l < Nh) {
+                      zr[l] = sp.re[int(t)];
+                      zr[64 + l] = sp.im[int(t)];
+                      zr[32 + l] = sm.re[int(t)];
+                      zr[96 + l] = sm.im[int(t)];
+                    }
+                  });
+                }
               }
             }
             if constexpr (X == 0) {
@@ -598,7 +623,12 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
       wave_lds_fence();
       if (lane < N) {
         const int lp = (lane == 0) ? 0 : N - lane;
-        const float2 z1 = zx[lane], q1 = zx[lp], z2 = zx[64 + lane], q2 = zx[64 + lp];
+        // rows l and -l in the planar layout (l >= Nh at 32 + l - Nh)
+        const int il = (lane < Nh) ? lane : lane - Nh + 32;
+        const int ip = (lp < Nh) ? lp : lp - Nh + 32;
+        const float2 z1 = make_float2(zx[il], zx[64 + il]), q1 = make_float2(zx[ip], zx[64 + ip]);
+        const float2 z2 = make_float2(zx[128 + il], zx[192 + il]);
+        const float2 q2 = make_float2(zx[128 + ip], zx[192 + ip]);
         const float a0r1 = 0.5f * (z1.x + q1.x), a0i1 = 0.5f * (z1.y - q1.y);
         const float anr1 = 0.5f * (z1.y + q1.y), ani1 = -0.5f * (z1.x - q1.x);
         const float a0r2 = 0.5f * (z2.x + q2.x), a0i2 = 0.5f * (z2.y - q2.y);
@@ -613,21 +643,30 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
         const float p0i = (a0r1 * a0i2 - a0i1 * a0r2) * n0f;
         const float pnr = (anr1 * anr2 + ani1 * ani2) * nnf;
         const float pni = (anr1 * ani2 - ani1 * anr2) * nnf;
-        zp[lane] = make_float2(p0r - pni, p0i + pnr);
+        zp[il] = p0r - pni;
+        zp[64 + il] = p0i + pnr;
       }
       wave_lds_fence();
       if (n == 0) {
         static_for<TL>([&](auto LT) {
-          static_for<4>([&](auto t) {
-            const int l = 16 * LT + 4 * g + t;
-            if (l < Nh) {
-              const float2 pp = zp[l], pm = zp[l + Nh];
-              Pp[0][LT].re[int(t)] = pp.x;
-              Pp[0][LT].im[int(t)] = pp.y;
-              Pm[0][LT].re[int(t)] = pm.x;
-              Pm[0][LT].im[int(t)] = pm.y;
-            }
-          });
+          const int l0 = 16 * LT + 4 * g;
+          if (l0 + 3 < Nh) {
+            // a whole group of four rows: one vector load per part
+            Pp[0][LT].re = *reinterpret_cast<const f32x4*>(zp + l0);
+            Pp[0][LT].im = *reinterpret_cast<const f32x4*>(zp + 64 + l0);
+            Pm[0][LT].re = *reinterpret_cast<const f32x4*>(zp + 32 + l0);
+            Pm[0][LT].im = *reinterpret_cast<const f32x4*>(zp + 96 + l0);
+          } else {
+            static_for<4>([&](auto t) {
+              const int l = l0 + t;
+              if (l < Nh) {
+                Pp[0][LT].re[int(t)] = zp[l];
+                Pp[0][LT].im[int(t)] = zp[64 + l];
+                Pm[0][LT].re[int(t)] = zp[32 + l];
+                Pm[0][LT].im[int(t)] = zp[96 + l];
+              }
+            });
+          }
         });
       }
       wave_lds_fence();
