@@ -290,14 +290,17 @@ def main():
     log(f"bench: end to end {e2e[-1]:.1f} ms")
     dtype = {"fp64": "f64", "fp32": "f32", "split_bf16": "bf16x3+f32acc"}[head]
 
+    # the kernel family of the path the headline form ran
+    fam = {"fp64-rowseg": "k_track_rs", "fp32-rowseg": "k_track_rs"}.get(hr["path"], "k_track")
     if head == "fp64":
-        peak, per_px, kname = PEAK_FP64_TFLOPS, w_fft(nbox), f"k_track<{nbox // 4},double>"
+        peak, per_px, kname = PEAK_FP64_TFLOPS, w_fft(nbox), f"{fam}<{nbox // 4},double>"
     elif head == "fp32":
-        peak, per_px, kname = PEAK_FP32_TFLOPS, w_fft(nbox), f"k_track<{nbox // 4},float>"
+        peak, per_px, kname = PEAK_FP32_TFLOPS, w_fft(nbox), f"{fam}<{nbox // 4},float>"
     else:
         peak, per_px = PEAK_BF16_TFLOPS, w_gemm(nbox)
-        kname = ("k_track_seg (interior columns) + k_track_r2<2> (edge columns)"
-                 if 36 <= nbox <= 64 else "k_track_mfma")
+        kname = ("k_track_seg (interior columns and the two transposed edge strips)"
+                 " + k_track_r2<2> (corners)"
+                 if 36 <= nbox <= 64 and hr["path"] == "bf16-segment" else "k_track_mfma")
     seg_basis = head == "split_bf16" and 36 <= nbox <= 64
     if seg_basis:
         # the contractions the row-segment algorithm needs (radix-2 column

@@ -887,11 +887,6 @@ __global__ void __launch_bounds__(SegCfg::kThreads, 1) k_track_seg(SegArgs A0) {
 }
 
 // Edge pixels for the per-pixel kernel: tracked pixels whose box is clamped
-// at the left or right image edge (columns outside [jlo, jhi]).  With a
-// tracked-pixel list (threshold or skip; its length in *nlist) the list is
-// filtered into out / *count by a grid-stride loop; otherwise the edge
-// columns of every row are enumerated.
-// Edge pixels for the per-pixel kernel: tracked pixels whose box is clamped
 // at the left or right image edge (columns outside [jlo, jhi]) and whose row
 // is outside [ilo, ihi] (rows inside it run through the transposed edge
 // strips; an empty range, ilo > ihi, keeps every row).  With a tracked-pixel
