@@ -113,6 +113,7 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
   unsigned char* base = reinterpret_cast<unsigned char*>(colst + 4 * Cfg::kColStat) +
                         (in_slot ? slot : 0) * Cfg::kSlotBytes;
   cplx<T>* Z = reinterpret_cast<cplx<T>*>(base);
+  // N doubles: with whole-wave slots the per-wave box extrema words
   double* wy = reinterpret_cast<double*>(base + Cfg::kZBytes);
   double* nb = wy + N;
   double* res = nb + 9;
@@ -240,6 +241,12 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
     __syncthreads();
 
     // ------------- the segment's tracked pixels, one per slot at a time
+    // Where a slot is one or two whole waves (boxes 32 and 64), each wave
+    // reduces the box extrema of its slot itself, into its own four LDS words
+    // (in the otherwise unused wy area), so neither the set-up nor the extrema
+    // need a block barrier; the column window is read from gw directly.
+    constexpr bool kWaveExt = (Cfg::kSlotThreads % 64 == 0) && N <= 64;
+    unsigned long long* bstw = reinterpret_cast<unsigned long long*>(wy) + 4 * (stid >> 6);
     for (int rp = 0; rp < cnt; rp += SP) {
       const bool active = in_slot && (rp + slot) < cnt;
       int j = 0, jmin = 0;
@@ -251,7 +258,14 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
         f2b = a.mean2[(long long)imin * ns + jmin];
         d1 = f1b - mref1;
         d2 = f2b - mref2;
-        if (stid < N) wy[stid] = gw[stid + jmin - j + N - 1];
+        if constexpr (kWaveExt) {
+          if ((stid & 63) == 0) {
+            bstw[0] = ~0ull;
+            bstw[1] = 0ull;
+            bstw[2] = ~0ull;
+            bstw[3] = 0ull;
+          }
+        }
         if (stid == 0) {
           *amax = 0ull;
           *aidx = 0xffffffffu;
@@ -262,10 +276,16 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
           *ntie = 0u;
         }
       }
-      __syncthreads();
+      if constexpr (kWaveExt) {
+        wave_lds_order();
+      } else {
+        __syncthreads();
+      }
       // exact box minimum and maximum of both images from the column values
-      if (active && stid < N) {
-        const int u = jmin - cb + stid;
+      // (with kWaveExt in every wave of the slot, else in its first N lanes)
+      unsigned long long* bs = kWaveExt ? bstw : bst;
+      if (active && (kWaveExt ? (stid & 63) : stid) < N) {
+        const int u = jmin - cb + (kWaveExt ? (stid & 63) : stid);
         double mn1 = colst[0 * Cfg::kColStat + u], mx1 = colst[1 * Cfg::kColStat + u];
         double mn2 = colst[2 * Cfg::kColStat + u], mx2 = colst[3 * Cfg::kColStat + u];
         mn1 = fmin(mn1, dpp<kQuadXor1>(mn1));
@@ -277,18 +297,22 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
         mn2 = fmin(mn2, dpp<kQuadXor2>(mn2));
         mx2 = fmax(mx2, dpp<kQuadXor2>(mx2));
         if ((stid & 3) == 0) {
-          atomicMin(&bst[0], hires_key(mn1));
-          atomicMax(&bst[1], hires_key(mx1));
-          atomicMin(&bst[2], hires_key(mn2));
-          atomicMax(&bst[3], hires_key(mx2));
+          atomicMin(&bs[0], hires_key(mn1));
+          atomicMax(&bs[1], hires_key(mx1));
+          atomicMin(&bs[2], hires_key(mn2));
+          atomicMax(&bs[3], hires_key(mx2));
         }
       }
-      __syncthreads();
+      if constexpr (kWaveExt) {
+        wave_lds_order();
+      } else {
+        __syncthreads();
+      }
       bool zero = false, exact = false;  // zero surface; re-evaluate in k_track
       double s1 = 1., s2 = 1.;             // FP32: image scales
       if (active) {
-        const double mn1 = key_value(bst[0]), mx1 = key_value(bst[1]);
-        const double mn2 = key_value(bst[2]), mx2 = key_value(bst[3]);
+        const double mn1 = key_value(bs[0]), mx1 = key_value(bs[1]);
+        const double mn2 = key_value(bs[2]), mx2 = key_value(bs[3]);
         zero = (mn1 == mx1 && mx1 == f1b) || (mn2 == mx2 && mx2 == f2b);
         const double o1 = fmax(fabs(mx1 - mref1), fabs(mn1 - mref1));
         const double o2 = fmax(fabs(mx2 - mref2), fabs(mn2 - mref2));
@@ -327,7 +351,7 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
           const double2 f = phi[(size_t)k * us + c0 + c];
           const cplx<double> w = wrh[k];
           const double cr = d1 * w.re - d2 * w.im, ci = d1 * w.im + d2 * w.re;
-          const double wc = wy[c];
+          const double wc = gw[c + jmin - j + N - 1];
           return {wc * (f.x - cr), wc * (f.y - ci)};
         };
         if (t >= 1) {
