@@ -247,7 +247,57 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
     // need a block barrier; the column window is read from gw directly.
     constexpr bool kWaveExt = (Cfg::kSlotThreads % 64 == 0) && N <= 64;
     unsigned long long* bstw = reinterpret_cast<unsigned long long*>(wy) + 4 * (stid >> 6);
+    // The 3 x 3 fit and the outputs of a slot's pixel are done by one lane
+    // of the slot (lane 64 where the slot has more than one wave: its first
+    // wave carries the rows 0 / N/2 task), at the start of the slot's next
+    // pixel or after its last one: the lane
+    // keeps the peak data in registers, nb is not rewritten before the next
+    // pixel's gather, and no block barrier waits on the single-lane FP64 work.
+    constexpr int kFitLane = (Cfg::kSlotThreads >= 128) ? 64 : 0;
+    bool pend = false, pexact = false, pfit = false, pflag = false;
+    int pj = 0, pix = N / 2, piy = N / 2;
+    double pgmax = 0.;
+    auto finish = [&]() {
+#pragma clang fp contract(off)
+      double shiftsubx = 0., shiftsuby = 0.;
+      bool flagged = pflag;
+      if (pfit) {
+        // nb[(dx+1)*3 + (dy+1)] = surface at (ixmax+dx, iymax+dy)
+        const double c00 = nb[4];
+        const double fx = 0.5 * (nb[7] - nb[1]);
+        const double fy = 0.5 * (nb[5] - nb[3]);
+        const double fxx = (nb[7] + nb[1] - 2. * (c00));
+        const double fyy = (nb[5] + nb[3] - 2. * (c00));
+        const double fxy = 0.25 * (nb[8] + nb[0] - nb[6] - nb[2]);
+        shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
+        shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
+        if constexpr (sizeof(T) == 4) {
+          // first-order response of the fit to relative errors kRelErr32
+          // of the nine values
+          const double det = fxx * fyy - fxy * fxy;
+          const double hn = fabs(fxx) + fabs(fyy) + 2. * fabs(fxy);
+          const double bound =
+              kRelErr32 * pgmax * hn * (1. + 4. * (fabs(shiftsubx) + fabs(shiftsuby)));
+          if (!(bound <= kFitTol32 * fabs(det))) flagged = true;
+        }
+      }
+      const double shiftx = ((double)pix + shiftsubx) - (double)(N / 2);
+      const double shifty = ((double)piy + shiftsuby) - (double)(N / 2);
+      const long long p = (long long)i * ny + pj;
+      if (pexact) {
+        a.flag_list[atomicAdd(a.flag_count, 1u)] = (int)p;  // k_track writes this pixel
+      } else {
+        a.vx[p] = shifty * a.deltinv * a.deltas;  // transp = 1 (flct.c:901-905)
+        a.vy[p] = shiftx * a.deltinv * a.deltas;
+        a.vm[p] = 1.;
+        if (flagged) a.flag_list[atomicAdd(a.flag_count, 1u)] = (int)p;  // FP32: re-evaluated
+      }
+    };
     for (int rp = 0; rp < cnt; rp += SP) {
+      if (pend) {
+        finish();
+        pend = false;
+      }
       const bool active = in_slot && (rp + slot) < cnt;
       int j = 0, jmin = 0;
       double d1 = 0., d2 = 0., f1b = 0., f2b = 0.;
@@ -505,47 +555,21 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
           }
         });
       }
-      __syncthreads();
-      if (active && stid == 0) {
-#pragma clang fp contract(off)
-        double shiftsubx = 0., shiftsuby = 0.;
-        bool flagged = false;
-        if (fit) {
-          // nb[(dx+1)*3 + (dy+1)] = surface at (ixmax+dx, iymax+dy)
-          const double c00 = nb[4];
-          const double fx = 0.5 * (nb[7] - nb[1]);
-          const double fy = 0.5 * (nb[5] - nb[3]);
-          const double fxx = (nb[7] + nb[1] - 2. * (c00));
-          const double fyy = (nb[5] + nb[3] - 2. * (c00));
-          const double fxy = 0.25 * (nb[8] + nb[0] - nb[6] - nb[2]);
-          shiftsubx = (fyy * fx - fy * fxy) / (fxy * fxy - fxx * fyy);
-          shiftsuby = (fxx * fy - fx * fxy) / (fxy * fxy - fxx * fyy);
-          if constexpr (sizeof(T) == 4) {
-            // first-order response of the fit to relative errors kRelErr32
-            // of the nine values
-            const double det = fxx * fyy - fxy * fxy;
-            const double hn = fabs(fxx) + fabs(fyy) + 2. * fabs(fxy);
-            const double bound =
-                kRelErr32 * gmax * hn * (1. + 4. * (fabs(shiftsubx) + fabs(shiftsuby)));
-            if (!(bound <= kFitTol32 * fabs(det))) flagged = true;
-          }
-        }
-        if constexpr (sizeof(T) == 4) {
-          if (nonzero && *ntie > 1u) flagged = true;
-        }
-        const double shiftx = ((double)ixmax + shiftsubx) - (double)(N / 2);
-        const double shifty = ((double)iymax + shiftsuby) - (double)(N / 2);
-        const long long p = (long long)i * ny + j;
-        if (exact) {
-          a.flag_list[atomicAdd(a.flag_count, 1u)] = (int)p;  // k_track writes this pixel
-        } else {
-          a.vx[p] = shifty * a.deltinv * a.deltas;  // transp = 1 (flct.c:901-905)
-          a.vy[p] = shiftx * a.deltinv * a.deltas;
-          a.vm[p] = 1.;
-          if (flagged) a.flag_list[atomicAdd(a.flag_count, 1u)] = (int)p;  // FP32: re-evaluated
-        }
+      if (active && stid == kFitLane) {
+        pend = true;
+        pj = j;
+        pexact = exact;
+        pfit = fit;
+        pix = ixmax;
+        piy = iymax;
+        pgmax = gmax;
+        if constexpr (sizeof(T) == 4) pflag = nonzero && *ntie > 1u;
       }
       __syncthreads();  // Z, nb and the maxima are rewritten by the next pixel
+    }
+    if (pend) {
+      finish();
+      pend = false;
     }
   }
 }
