@@ -404,46 +404,33 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
           const double wc = gw[c + jmin - j + N - 1];
           return {wc * (f.x - cr), wc * (f.y - ci)};
         };
-        if (t >= 1) {
-          const int k = t, km = N - t;
-          // A = (Zk + conj(Zm))/2 and B = (Zk - conj(Zm))/(2i) from one read
-          // of Phi rows k and N - k; both spectra stay in registers
-          cplx<T> u[M], v[M];
-          static_for<M>([&](auto Rr) {
-            const int jj = 4 * Rr + l;
-            const cplx<double> zk = zval(k, jj), zm = zval(km, jj);
-            u[Rr] = {tv(0.5 * (zk.re + zm.re), s1), tv(0.5 * (zk.im - zm.im), s1)};
-            v[Rr] = {tv(0.5 * (zk.im + zm.im), s2), tv(-0.5 * (zk.re - zm.re), s2)};
-          });
-          quad_dit<M, -1>(u, l);
-          quad_dit<M, -1>(v, l);
-          static_for<M>([&](auto S) {
-            cplx<T> Aa = u[S], B = v[S];
-            if (a.filt) {
-              const T f = T(a.filt[k * N + lb + S]);
-              Aa = {Aa.re * f, Aa.im * f};
-              B = {B.re * f, B.im * f};
-            }
-            u[S] = cross_power(Aa, B, norm);
-          });
-          quad_dif<M, +1>(u, l);
-          static_for<M>([&](auto Rr) { Z[zidx<M>(k, 4 * Rr + l)] = u[Rr]; });
-        } else {
-          // rows k = 0 and k = N/2 of A and B are real along jj: one packed
-          // transform per image, parked in rows N/2 (A) and 0 (B)
-          cplx<T> u[M], v[M];
-          static_for<M>([&](auto Rr) {
-            const int jj = 4 * Rr + l;
-            const cplx<double> z0 = zval(0, jj), zh = zval(N / 2, jj);
-            u[Rr] = {tv(z0.re, s1), tv(zh.re, s1)};
-            v[Rr] = {tv(z0.im, s2), tv(zh.im, s2)};
-          });
-          quad_dit<M, -1>(u, l);
+        // Row task t >= 1: rows k = t and N - t; A = (Zk + conj(Zm))/2 and
+        // B = (Zk - conj(Zm))/(2i) from one read of the two Phi rows, both
+        // spectra in registers.  Task 0: rows 0 and N/2 of A and B are real
+        // along jj, so each image is one packed transform (row 0 + i row
+        // N/2), unpacked after the forward transforms through LDS.  The
+        // transforms are shared code for every task: only the input, the
+        // unpack and the stores depend on it, so the wave holding task 0
+        // does not run the transforms twice.
+        const bool tz = (t == 0);
+        const int k = tz ? 0 : t, km = tz ? N / 2 : N - t;
+        cplx<T> u[M], v[M];
+        static_for<M>([&](auto Rr) {
+          const int jj = 4 * Rr + l;
+          const cplx<double> zk = zval(k, jj), zm = zval(km, jj);
+          const cplx<T> ug = {tv(0.5 * (zk.re + zm.re), s1), tv(0.5 * (zk.im - zm.im), s1)};
+          const cplx<T> vg = {tv(0.5 * (zk.im + zm.im), s2), tv(-0.5 * (zk.re - zm.re), s2)};
+          const cplx<T> up = {tv(zk.re, s1), tv(zm.re, s1)};
+          const cplx<T> vp = {tv(zk.im, s2), tv(zm.im, s2)};
+          u[Rr] = tz ? up : ug;
+          v[Rr] = tz ? vp : vg;
+        });
+        quad_dit<M, -1>(u, l);
+        quad_dit<M, -1>(v, l);
+        if (tz) {
+          // the packed transforms parked in rows N/2 (A) and 0 (B)
           wave_lds_order();
           static_for<M>([&](auto S) { Z[zidx<M>(N / 2, 4 * S + l)] = u[S]; });
-          wave_lds_order();
-          quad_dit<M, -1>(v, l);
-          wave_lds_order();
           static_for<M>([&](auto S) { Z[zidx<M>(0, 4 * S + l)] = v[S]; });
           wave_lds_order();
           static_for<M>([&](auto S) {
@@ -468,13 +455,27 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
             const cplx<T> P0 = cross_power(A0, B0, norm), Ph = cross_power(Ah, Bh, norm);
             u[S] = {P0.re - Ph.im, P0.im + Ph.re};
           });
-          quad_dif<M, +1>(u, l);
+        } else {
+          static_for<M>([&](auto S) {
+            cplx<T> Aa = u[S], B = v[S];
+            if (a.filt) {
+              const T f = T(a.filt[k * N + lb + S]);
+              Aa = {Aa.re * f, Aa.im * f};
+              B = {B.re * f, B.im * f};
+            }
+            u[S] = cross_power(Aa, B, norm);
+          });
+        }
+        quad_dif<M, +1>(u, l);
+        if (tz) {
           wave_lds_order();
           static_for<M>([&](auto Rr) {
             const int jj = 4 * Rr + l;
             Z[zidx<M>(0, jj)] = {u[Rr].re, T(0)};
             Z[zidx<M>(N / 2, jj)] = {u[Rr].im, T(0)};
           });
+        } else {
+          static_for<M>([&](auto Rr) { Z[zidx<M>(k, 4 * Rr + l)] = u[Rr]; });
         }
       }
       __syncthreads();
