@@ -362,35 +362,37 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
 // kernel's cost is flat below box 32), the row-segment kernel from 20 (box
 // 20 with -b: FP32).  On boxes above 64 it runs the FP32 kernels (with their FP64 re-evaluation of
 // unsettled pixels), which are within the split-BF16 error bound and were
-// measured 29-41 % faster than the FP64 kernels and 2.9-3.7x faster than
-// the dense-DFT BF16 kernel there (2048^2, boxes 68..84).  Among the FP32
+// measured 1.9-3.1x faster than the FP64 kernels there (2048^2, boxes 68..84:
+// 133-161 ms against 257-493 ms) and 2.9-3.7x faster than the dense-DFT
+// BF16 kernel even before their shared row transforms.  Among the FP32
 // kernels the row-shared one is taken where it was faster (rowseg32_default).
 // Box sizes where a row-shared kernel was measured faster than the quad-FFT
 // kernel of its precision by 4 % or more (2048^2 full-disk pair, every pixel
 // tracked, whole-call ms, quad FFT / row-shared).
-// FP64: 8 3.1/2.5, 12 5.9/4.9, 16 7.7/7.1, 28 28.8/27.4, 32 31.2/26.8,
-// 36 56.5/39.6, 40 80.7/48.7, 44 106/72.2, 48 125/79.2, 52 174/125,
-// 60 203/140, 64 212/143, 68 449/332, 72 442/335, 76 513/387, 80 497/370,
-// 84 549/417, 88 589/496, 96 709/539; level or slower at 4, 20, 24 and 92.
+// FP64: 8 3.1/2.7, 12 5.9/4.4, 16 7.6/6.1, 24 22.3/18.4, 28 28.8/22.2,
+// 32 31.3/22.2, 36 56.3/32.3, 40 80.7/39.4, 44 106/55.4, 48 125/80.2,
+// 52 173/101, 60 203/113, 64 212/114, 68 447/257, 72 442/261, 76 513/309,
+// 80 497/404, 84 549/493, 88 588/476, 92 712/635, 96 701/561; level or
+// slower at 4 and 20.
 static bool rowseg_default(int N) {
   switch (N) {
-    case 8: case 12: case 16: case 28: case 32: case 36: case 40: case 44: case 48: case 52: case 56:
-    case 60: case 64: case 68: case 72: case 76: case 80: case 84: case 88: case 96:
+    case 8: case 12: case 16: case 24: case 28: case 32: case 36: case 40: case 44: case 48: case 52:
+    case 56: case 60: case 64: case 68: case 72: case 76: case 80: case 84: case 88: case 92: case 96:
       return true;
     default:
       return false;
   }
 }
-// FP32 row tasks (fp32-rowseg against fp32-fft): 8 1.9/2.9, 12 4.1/4.7,
-// 16 5.8/7.2, 32 25.4/30.5, 36 37.1/53.7, 40 46.3/59.1, 44 62.6/100.4,
-// 48 64.5/110.4, 52 95.9/148.8, 60 115.7/142.4, 64 113.9/165.4, 68 238/266,
-// 72 204/287, 76 284/328, 80 242/343, 84 302/382, 96 400/446; level or
-// slower at 4, 20..28, 88 and 92.  Used by the FP32 form and by the
-// split-BF16 form on the boxes it runs in FP32.
+// FP32 row tasks (fp32-fft / fp32-rowseg): 8 2.9/1.7, 12 4.7/3.6,
+// 16 7.2/5.1, 24 18.7/16.8, 28 25.0/21.1, 32 30.5/21.0, 36 53.5/30.9,
+// 40 59.1/37.2, 44 100/51.0, 48 110/50.0, 52 148/77.7, 60 143/74.9,
+// 64 165/88.0, 68 265/133, 72 286/132, 76 329/157, 80 342/144, 84 385/161,
+// 88 387/211, 92 453/246, 96 452/228; level or slower at 4 and 20.  Used by
+// the FP32 form and by the split-BF16 form on the boxes it runs in FP32.
 static bool rowseg32_default(int N) {
   switch (N) {
-    case 8: case 12: case 16: case 32: case 36: case 40: case 44: case 48: case 52: case 56:
-    case 60: case 64: case 68: case 72: case 76: case 80: case 84: case 96:
+    case 8: case 12: case 16: case 24: case 28: case 32: case 36: case 40: case 44: case 48: case 52:
+    case 56: case 60: case 64: case 68: case 72: case 76: case 80: case 84: case 88: case 92: case 96:
       return true;
     default:
       return false;
