@@ -484,51 +484,34 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads), (TrackCfg<M, T>::k
         const int l = fresh(lane), quad = fresh(qd);
         const int t = quad;
         const int lb = M * brev2(l);
-        if (t >= 1) {
-          const int k = t, km = N - t;
-          cplx<T> u1[M], u2[M];
-          static_for<M>([&](auto Rr) {
-            const int jj = 4 * Rr + l;
-            const cplx<T> zk = Z[zidx<M>(k, jj)], zm = Z[zidx<M>(km, jj)];
-            // A = (Zk + conj(Zm))/2, B = (Zk - conj(Zm))/(2i)
-            u1[Rr] = {T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
-            u2[Rr] = {T(0.5) * (zk.im + zm.im), T(-0.5) * (zk.re - zm.re)};
-          });
-          quad_dit<M, -1>(u1, l);
-          // park A's spectrum in the mirror row, which only this quad reads
-          wave_lds_order();
-          static_for<M>([&](auto S) { Z[zidx<M>(km, 4 * S + l)] = u1[S]; });
-          wave_lds_order();
-          quad_dit<M, -1>(u2, l);
-          wave_lds_order();
-          static_for<M>([&](auto S) {
-            cplx<T> A = Z[zidx<M>(km, 4 * S + l)], B = u2[S];
-            if (a.filt) {
-              const T f = T(a.filt[k * N + lb + S]);
-              A = {A.re * f, A.im * f};
-              B = {B.re * f, B.im * f};
-            }
-            u2[S] = cross_power(A, B, norm);
-          });
-          quad_dif<M, +1>(u2, l);
-          static_for<M>([&](auto Rr) { Z[zidx<M>(k, 4 * Rr + l)] = u2[Rr]; });
-        } else {
-          // rows k = 0 and k = N/2 of A and B are real along jj: one packed
-          // transform per image, parked in rows N/2 (A) and 0 (B)
-          cplx<T> u[M], v[M];
-          static_for<M>([&](auto Rr) {
-            const int jj = 4 * Rr + l;
-            const cplx<T> z0 = Z[zidx<M>(0, jj)], zh = Z[zidx<M>(N / 2, jj)];
-            u[Rr] = {z0.re, zh.re};
-            v[Rr] = {z0.im, zh.im};
-          });
-          quad_dit<M, -1>(u, l);
-          wave_lds_order();
-          static_for<M>([&](auto S) { Z[zidx<M>(N / 2, 4 * S + l)] = u[S]; });
-          wave_lds_order();
-          quad_dit<M, -1>(v, l);
-          wave_lds_order();
-          static_for<M>([&](auto S) { Z[zidx<M>(0, 4 * S + l)] = v[S]; });
+        // Row task t >= 1: rows k = t and N - t, A = (Zk + conj(Zm))/2 and
+        // B = (Zk - conj(Zm))/(2i), A's spectrum parked in the mirror row,
+        // which only this quad reads.  Task 0: rows 0 and N/2 of A and B are
+        // real along jj, so each image is one packed transform (row 0 + i row
+        // N/2), parked in rows N/2 (A) and 0 (B) and unpacked through them.
+        // The transforms are shared code for every task (only the input, the
+        // unpack and the stores depend on it), so the wave holding task 0
+        // does not run them twice.
+        const bool tz = (t == 0);
+        const int k = tz ? 0 : t, km = tz ? N / 2 : N - t;
+        cplx<T> u1[M], u2[M];
+        static_for<M>([&](auto Rr) {
+          const int jj = 4 * Rr + l;
+          const cplx<T> zk = Z[zidx<M>(k, jj)], zm = Z[zidx<M>(km, jj)];
+          const cplx<T> g1 = {T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
+          const cplx<T> g2 = {T(0.5) * (zk.im + zm.im), T(-0.5) * (zk.re - zm.re)};
+          const cplx<T> p1 = {zk.re, zm.re}, p2 = {zk.im, zm.im};
+          u1[Rr] = tz ? p1 : g1;
+          u2[Rr] = tz ? p2 : g2;
+        });
+        quad_dit<M, -1>(u1, l);
+        wave_lds_order();
+        static_for<M>([&](auto S) { Z[zidx<M>(km, 4 * S + l)] = u1[S]; });
+        wave_lds_order();
+        quad_dit<M, -1>(u2, l);
+        wave_lds_order();
+        if (tz) {
+          static_for<M>([&](auto S) { Z[zidx<M>(0, 4 * S + l)] = u2[S]; });
           wave_lds_order();
           static_for<M>([&](auto S) {
             const int lx = lb + S;
@@ -550,15 +533,29 @@ __global__ void __launch_bounds__((TrackCfg<M, T>::kThreads), (TrackCfg<M, T>::k
               Bh = {Bh.re * fh, Bh.im * fh};
             }
             const cplx<T> P0 = cross_power(A0, B0, norm), Ph = cross_power(Ah, Bh, norm);
-            u[S] = {P0.re - Ph.im, P0.im + Ph.re};
+            u2[S] = {P0.re - Ph.im, P0.im + Ph.re};
           });
-          quad_dif<M, +1>(u, l);
+        } else {
+          static_for<M>([&](auto S) {
+            cplx<T> A = Z[zidx<M>(km, 4 * S + l)], B = u2[S];
+            if (a.filt) {
+              const T f = T(a.filt[k * N + lb + S]);
+              A = {A.re * f, A.im * f};
+              B = {B.re * f, B.im * f};
+            }
+            u2[S] = cross_power(A, B, norm);
+          });
+        }
+        quad_dif<M, +1>(u2, l);
+        if (tz) {
           wave_lds_order();
           static_for<M>([&](auto Rr) {
             const int jj = 4 * Rr + l;
-            Z[zidx<M>(0, jj)] = {u[Rr].re, T(0)};
-            Z[zidx<M>(N / 2, jj)] = {u[Rr].im, T(0)};
+            Z[zidx<M>(0, jj)] = {u2[Rr].re, T(0)};
+            Z[zidx<M>(N / 2, jj)] = {u2[Rr].im, T(0)};
           });
+        } else {
+          static_for<M>([&](auto Rr) { Z[zidx<M>(k, 4 * Rr + l)] = u2[Rr]; });
         }
       }
       __syncthreads();
