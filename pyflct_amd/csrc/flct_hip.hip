@@ -369,29 +369,30 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
 // Box sizes where a row-shared kernel was measured faster than the quad-FFT
 // kernel of its precision by 4 % or more (2048^2 full-disk pair, every pixel
 // tracked, whole-call ms, quad FFT / row-shared).
-// FP64: 8 3.1/2.7, 12 5.9/4.4, 16 7.6/6.1, 24 22.3/18.4, 28 28.8/22.2,
-// 32 31.3/22.2, 36 56.3/32.3, 40 80.7/39.4, 44 106/55.4, 48 125/80.2,
-// 52 173/101, 60 203/113, 64 212/114, 68 447/257, 72 442/261, 76 513/309,
-// 80 497/404, 84 549/493, 88 588/476, 92 712/635, 96 701/561; level or
-// slower at 4 and 20.
+// FP64: 12 5.3/4.4, 16 6.8/6.1, 24 20.3/18.4, 28 28.8/22.2, 32 27.1/22.2,
+// 40 77.9/39.4, 44 106/55.4, 48 118/80.2, 52 173/101, 60 180/113,
+// 64 195/114, 68 389/257, 72 390/261, 76 435/309, 80 431/404, 88 588/476,
+// 96 616/561; 8 2.8/2.7 (kept); level or slower at 4 (1.1/1.7), 20
+// (13.6/16.2), 84 (478/493) and 92 (609/635).
 static bool rowseg_default(int N) {
   switch (N) {
     case 8: case 12: case 16: case 24: case 28: case 32: case 36: case 40: case 44: case 48: case 52:
-    case 56: case 60: case 64: case 68: case 72: case 76: case 80: case 84: case 88: case 92: case 96:
+    case 56: case 60: case 64: case 68: case 72: case 76: case 80: case 88: case 96:
       return true;
     default:
       return false;
   }
 }
-// FP32 row tasks (fp32-fft / fp32-rowseg): 8 2.9/1.7, 12 4.7/3.6,
-// 16 7.2/5.1, 24 18.7/16.8, 28 25.0/21.1, 32 30.5/21.0, 36 53.5/30.9,
-// 40 59.1/37.2, 44 100/51.0, 48 110/50.0, 52 148/77.7, 60 143/74.9,
-// 64 165/88.0, 68 265/133, 72 286/132, 76 329/157, 80 342/144, 84 385/161,
-// 88 387/211, 92 453/246, 96 452/228; level or slower at 4 and 20.  Used by
-// the FP32 form and by the split-BF16 form on the boxes it runs in FP32.
+// FP32 row tasks (fp32-fft / fp32-rowseg): 8 2.7/1.7, 12 4.3/3.6,
+// 16 6.8/5.1, 28 25.0/21.1, 32 27.7/21.0, 36 53.5/30.9, 40 53.9/37.2,
+// 44 100/51.0, 48 104/50.0, 52 148/77.7, 60 134/74.9, 64 155/88.0,
+// 68 243/133, 72 258/132, 76 291/157, 80 316/144, 84 352/161, 88 387/211,
+// 92 411/246, 96 427/228; level or slower at 4 (1.1/1.1), 20 (11.6/15.1)
+// and 24 (16.7/16.8).  Used by the FP32 form and by the split-BF16 form on
+// the boxes it runs in FP32.
 static bool rowseg32_default(int N) {
   switch (N) {
-    case 8: case 12: case 16: case 24: case 28: case 32: case 36: case 40: case 44: case 48: case 52:
+    case 8: case 12: case 16: case 28: case 32: case 36: case 40: case 44: case 48: case 52:
     case 56: case 60: case 64: case 68: case 72: case 76: case 80: case 84: case 88: case 92: case 96:
       return true;
     default:

@@ -299,7 +299,7 @@ def test_paths_and_autotune():
 BOXES = list(range(4, 100, 4))
 # boxes whose untuned FP32 kernel (FP32 form; split-BF16 outside 24..64) is
 # the row-shared one (fp32-rowseg) when no threshold is given
-FP32_ROWSEG_BOXES = {8, 12, 16, 24, 28, 32, 36, 40, 44, 48, 52, 56, 60, 64, 68, 72, 76, 80, 84, 88, 92, 96}
+FP32_ROWSEG_BOXES = {8, 12, 16, 28, 32, 36, 40, 44, 48, 52, 56, 60, 64, 68, 72, 76, 80, 84, 88, 92, 96}
 
 
 def _box_case(nbox):
