@@ -371,13 +371,13 @@ static std::vector<int> path_candidates(int precision, int N, int nx, int ny, in
 // tracked, whole-call ms, quad FFT / row-shared).
 // FP64: 12 5.3/4.4, 16 6.8/6.1, 24 20.3/18.4, 28 28.8/22.2, 32 27.1/22.2,
 // 40 77.9/39.4, 44 106/55.4, 48 118/80.2, 52 173/101, 60 180/113,
-// 64 195/114, 68 389/257, 72 390/261, 76 435/309, 80 431/404, 88 588/476,
-// 96 616/561; 8 2.8/2.7 (kept); level or slower at 4 (1.1/1.7), 20
-// (13.6/16.2), 84 (478/493) and 92 (609/635).
+// 64 195/114, 68 389/257, 72 390/261, 76 431/309, 80 428/371, 84 474/418,
+// 88 588/476, 96 616/561; 8 2.8/2.7 (kept); level or slower at 4
+// (1.1/1.7), 20 (13.6/16.2) and 92 (609/635).
 static bool rowseg_default(int N) {
   switch (N) {
     case 8: case 12: case 16: case 24: case 28: case 32: case 36: case 40: case 44: case 48: case 52:
-    case 56: case 60: case 64: case 68: case 72: case 76: case 80: case 88: case 96:
+    case 56: case 60: case 64: case 68: case 72: case 76: case 80: case 84: case 88: case 96:
       return true;
     default:
       return false;

@@ -254,6 +254,7 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
     // keeps the peak data in registers, nb is not rewritten before the next
     // pixel's gather, and no block barrier waits on the single-lane FP64 work.
     constexpr int kFitLane = (Cfg::kSlotThreads >= 128) ? 64 : 0;
+    constexpr bool kBranchTasks = sizeof(T) == 8 && (M == 20 || M == 21);
     bool pend = false, pexact = false, pfit = false, pflag = false;
     int pj = 0, pix = N / 2, piy = N / 2;
     double pgmax = 0.;
@@ -404,78 +405,157 @@ __global__ void __launch_bounds__((RsCfg<M, T>::kThreads), (RsCfg<M, T>::kMinWav
           const double wc = gw[c + jmin - j + N - 1];
           return {wc * (f.x - cr), wc * (f.y - ci)};
         };
-        // Row task t >= 1: rows k = t and N - t; A = (Zk + conj(Zm))/2 and
-        // B = (Zk - conj(Zm))/(2i) from one read of the two Phi rows, both
-        // spectra in registers.  Task 0: rows 0 and N/2 of A and B are real
-        // along jj, so each image is one packed transform (row 0 + i row
-        // N/2), unpacked after the forward transforms through LDS.  The
-        // transforms are shared code for every task: only the input, the
-        // unpack and the stores depend on it, so the wave holding task 0
-        // does not run the transforms twice.
-        const bool tz = (t == 0);
-        const int k = tz ? 0 : t, km = tz ? N / 2 : N - t;
-        cplx<T> u[M], v[M];
-        static_for<M>([&](auto Rr) {
-          const int jj = 4 * Rr + l;
-          const cplx<double> zk = zval(k, jj), zm = zval(km, jj);
-          const cplx<T> ug = {tv(0.5 * (zk.re + zm.re), s1), tv(0.5 * (zk.im - zm.im), s1)};
-          const cplx<T> vg = {tv(0.5 * (zk.im + zm.im), s2), tv(-0.5 * (zk.re - zm.re), s2)};
-          const cplx<T> up = {tv(zk.re, s1), tv(zm.re, s1)};
-          const cplx<T> vp = {tv(zk.im, s2), tv(zm.im, s2)};
-          u[Rr] = tz ? up : ug;
-          v[Rr] = tz ? vp : vg;
-        });
-        quad_dit<M, -1>(u, l);
-        quad_dit<M, -1>(v, l);
-        if (tz) {
-          // the packed transforms parked in rows N/2 (A) and 0 (B)
-          wave_lds_order();
-          static_for<M>([&](auto S) { Z[zidx<M>(N / 2, 4 * S + l)] = u[S]; });
-          static_for<M>([&](auto S) { Z[zidx<M>(0, 4 * S + l)] = v[S]; });
-          wave_lds_order();
-          static_for<M>([&](auto S) {
-            const int lx = lb + S;
-            const int lm = (N - lx) % N;
-            const cplx<T> Uu = Z[zidx<M>(N / 2, park_col<M>(lx))];
-            const cplx<T> Um = Z[zidx<M>(N / 2, park_col<M>(lm))];
-            const cplx<T> V = Z[zidx<M>(0, park_col<M>(lx))];
-            const cplx<T> Vm = Z[zidx<M>(0, park_col<M>(lm))];
-            // row 0 = (U + conj(Um))/2, row N/2 = (U - conj(Um))/(2i)
-            cplx<T> A0 = {T(0.5) * (Uu.re + Um.re), T(0.5) * (Uu.im - Um.im)};
-            cplx<T> Ah = {T(0.5) * (Uu.im + Um.im), T(-0.5) * (Uu.re - Um.re)};
-            cplx<T> B0 = {T(0.5) * (V.re + Vm.re), T(0.5) * (V.im - Vm.im)};
-            cplx<T> Bh = {T(0.5) * (V.im + Vm.im), T(-0.5) * (V.re - Vm.re)};
-            if (a.filt) {
-              const T f0 = T(a.filt[0 * N + lx]), fh = T(a.filt[(N / 2) * N + lx]);
-              A0 = {A0.re * f0, A0.im * f0};
-              B0 = {B0.re * f0, B0.im * f0};
-              Ah = {Ah.re * fh, Ah.im * fh};
-              Bh = {Bh.re * fh, Bh.im * fh};
-            }
-            const cplx<T> P0 = cross_power(A0, B0, norm), Ph = cross_power(Ah, Bh, norm);
-            u[S] = {P0.re - Ph.im, P0.im + Ph.re};
-          });
+        // FP64 boxes 80 and 84 (one wave per SIMD, 200-226 accumulation
+        // registers): the shared-transform form below allocated worse there
+        // (404 and 493 ms at 2048^2 against 370 and 417 with a branch per
+        // task kind), so these two keep the branch per task kind.
+        if constexpr (kBranchTasks) {
+          if (t >= 1) {
+            const int k = t, km = N - t;
+            // A = (Zk + conj(Zm))/2 and B = (Zk - conj(Zm))/(2i) from one read
+            // of Phi rows k and N - k; both spectra stay in registers
+            cplx<T> u[M], v[M];
+            static_for<M>([&](auto Rr) {
+              const int jj = 4 * Rr + l;
+              const cplx<double> zk = zval(k, jj), zm = zval(km, jj);
+              u[Rr] = {tv(0.5 * (zk.re + zm.re), s1), tv(0.5 * (zk.im - zm.im), s1)};
+              v[Rr] = {tv(0.5 * (zk.im + zm.im), s2), tv(-0.5 * (zk.re - zm.re), s2)};
+            });
+            quad_dit<M, -1>(u, l);
+            quad_dit<M, -1>(v, l);
+            static_for<M>([&](auto S) {
+              cplx<T> Aa = u[S], B = v[S];
+              if (a.filt) {
+                const T f = T(a.filt[k * N + lb + S]);
+                Aa = {Aa.re * f, Aa.im * f};
+                B = {B.re * f, B.im * f};
+              }
+              u[S] = cross_power(Aa, B, norm);
+            });
+            quad_dif<M, +1>(u, l);
+            static_for<M>([&](auto Rr) { Z[zidx<M>(k, 4 * Rr + l)] = u[Rr]; });
+          } else {
+            // rows k = 0 and k = N/2 of A and B are real along jj: one packed
+            // transform per image, parked in rows N/2 (A) and 0 (B)
+            cplx<T> u[M], v[M];
+            static_for<M>([&](auto Rr) {
+              const int jj = 4 * Rr + l;
+              const cplx<double> z0 = zval(0, jj), zh = zval(N / 2, jj);
+              u[Rr] = {tv(z0.re, s1), tv(zh.re, s1)};
+              v[Rr] = {tv(z0.im, s2), tv(zh.im, s2)};
+            });
+            quad_dit<M, -1>(u, l);
+            wave_lds_order();
+            static_for<M>([&](auto S) { Z[zidx<M>(N / 2, 4 * S + l)] = u[S]; });
+            wave_lds_order();
+            quad_dit<M, -1>(v, l);
+            wave_lds_order();
+            static_for<M>([&](auto S) { Z[zidx<M>(0, 4 * S + l)] = v[S]; });
+            wave_lds_order();
+            static_for<M>([&](auto S) {
+              const int lx = lb + S;
+              const int lm = (N - lx) % N;
+              const cplx<T> Uu = Z[zidx<M>(N / 2, park_col<M>(lx))];
+              const cplx<T> Um = Z[zidx<M>(N / 2, park_col<M>(lm))];
+              const cplx<T> V = Z[zidx<M>(0, park_col<M>(lx))];
+              const cplx<T> Vm = Z[zidx<M>(0, park_col<M>(lm))];
+              // row 0 = (U + conj(Um))/2, row N/2 = (U - conj(Um))/(2i)
+              cplx<T> A0 = {T(0.5) * (Uu.re + Um.re), T(0.5) * (Uu.im - Um.im)};
+              cplx<T> Ah = {T(0.5) * (Uu.im + Um.im), T(-0.5) * (Uu.re - Um.re)};
+              cplx<T> B0 = {T(0.5) * (V.re + Vm.re), T(0.5) * (V.im - Vm.im)};
+              cplx<T> Bh = {T(0.5) * (V.im + Vm.im), T(-0.5) * (V.re - Vm.re)};
+              if (a.filt) {
+                const T f0 = T(a.filt[0 * N + lx]), fh = T(a.filt[(N / 2) * N + lx]);
+                A0 = {A0.re * f0, A0.im * f0};
+                B0 = {B0.re * f0, B0.im * f0};
+                Ah = {Ah.re * fh, Ah.im * fh};
+                Bh = {Bh.re * fh, Bh.im * fh};
+              }
+              const cplx<T> P0 = cross_power(A0, B0, norm), Ph = cross_power(Ah, Bh, norm);
+              u[S] = {P0.re - Ph.im, P0.im + Ph.re};
+            });
+            quad_dif<M, +1>(u, l);
+            wave_lds_order();
+            static_for<M>([&](auto Rr) {
+              const int jj = 4 * Rr + l;
+              Z[zidx<M>(0, jj)] = {u[Rr].re, T(0)};
+              Z[zidx<M>(N / 2, jj)] = {u[Rr].im, T(0)};
+            });
+          }
         } else {
-          static_for<M>([&](auto S) {
-            cplx<T> Aa = u[S], B = v[S];
-            if (a.filt) {
-              const T f = T(a.filt[k * N + lb + S]);
-              Aa = {Aa.re * f, Aa.im * f};
-              B = {B.re * f, B.im * f};
-            }
-            u[S] = cross_power(Aa, B, norm);
-          });
-        }
-        quad_dif<M, +1>(u, l);
-        if (tz) {
-          wave_lds_order();
+          // Row task t >= 1: rows k = t and N - t; A = (Zk + conj(Zm))/2 and
+          // B = (Zk - conj(Zm))/(2i) from one read of the two Phi rows, both
+          // spectra in registers.  Task 0: rows 0 and N/2 of A and B are real
+          // along jj, so each image is one packed transform (row 0 + i row
+          // N/2), unpacked after the forward transforms through LDS.  The
+          // transforms are shared code for every task: only the input, the
+          // unpack and the stores depend on it, so the wave holding task 0
+          // does not run the transforms twice.
+          const bool tz = (t == 0);
+          const int k = tz ? 0 : t, km = tz ? N / 2 : N - t;
+          cplx<T> u[M], v[M];
           static_for<M>([&](auto Rr) {
             const int jj = 4 * Rr + l;
-            Z[zidx<M>(0, jj)] = {u[Rr].re, T(0)};
-            Z[zidx<M>(N / 2, jj)] = {u[Rr].im, T(0)};
+            const cplx<double> zk = zval(k, jj), zm = zval(km, jj);
+            const cplx<T> ug = {tv(0.5 * (zk.re + zm.re), s1), tv(0.5 * (zk.im - zm.im), s1)};
+            const cplx<T> vg = {tv(0.5 * (zk.im + zm.im), s2), tv(-0.5 * (zk.re - zm.re), s2)};
+            const cplx<T> up = {tv(zk.re, s1), tv(zm.re, s1)};
+            const cplx<T> vp = {tv(zk.im, s2), tv(zm.im, s2)};
+            u[Rr] = tz ? up : ug;
+            v[Rr] = tz ? vp : vg;
           });
-        } else {
-          static_for<M>([&](auto Rr) { Z[zidx<M>(k, 4 * Rr + l)] = u[Rr]; });
+          quad_dit<M, -1>(u, l);
+          quad_dit<M, -1>(v, l);
+          if (tz) {
+            // the packed transforms parked in rows N/2 (A) and 0 (B)
+            wave_lds_order();
+            static_for<M>([&](auto S) { Z[zidx<M>(N / 2, 4 * S + l)] = u[S]; });
+            static_for<M>([&](auto S) { Z[zidx<M>(0, 4 * S + l)] = v[S]; });
+            wave_lds_order();
+            static_for<M>([&](auto S) {
+              const int lx = lb + S;
+              const int lm = (N - lx) % N;
+              const cplx<T> Uu = Z[zidx<M>(N / 2, park_col<M>(lx))];
+              const cplx<T> Um = Z[zidx<M>(N / 2, park_col<M>(lm))];
+              const cplx<T> V = Z[zidx<M>(0, park_col<M>(lx))];
+              const cplx<T> Vm = Z[zidx<M>(0, park_col<M>(lm))];
+              // row 0 = (U + conj(Um))/2, row N/2 = (U - conj(Um))/(2i)
+              cplx<T> A0 = {T(0.5) * (Uu.re + Um.re), T(0.5) * (Uu.im - Um.im)};
+              cplx<T> Ah = {T(0.5) * (Uu.im + Um.im), T(-0.5) * (Uu.re - Um.re)};
+              cplx<T> B0 = {T(0.5) * (V.re + Vm.re), T(0.5) * (V.im - Vm.im)};
+              cplx<T> Bh = {T(0.5) * (V.im + Vm.im), T(-0.5) * (V.re - Vm.re)};
+              if (a.filt) {
+                const T f0 = T(a.filt[0 * N + lx]), fh = T(a.filt[(N / 2) * N + lx]);
+                A0 = {A0.re * f0, A0.im * f0};
+                B0 = {B0.re * f0, B0.im * f0};
+                Ah = {Ah.re * fh, Ah.im * fh};
+                Bh = {Bh.re * fh, Bh.im * fh};
+              }
+              const cplx<T> P0 = cross_power(A0, B0, norm), Ph = cross_power(Ah, Bh, norm);
+              u[S] = {P0.re - Ph.im, P0.im + Ph.re};
+            });
+          } else {
+            static_for<M>([&](auto S) {
+              cplx<T> Aa = u[S], B = v[S];
+              if (a.filt) {
+                const T f = T(a.filt[k * N + lb + S]);
+                Aa = {Aa.re * f, Aa.im * f};
+                B = {B.re * f, B.im * f};
+              }
+              u[S] = cross_power(Aa, B, norm);
+            });
+          }
+          quad_dif<M, +1>(u, l);
+          if (tz) {
+            wave_lds_order();
+            static_for<M>([&](auto Rr) {
+              const int jj = 4 * Rr + l;
+              Z[zidx<M>(0, jj)] = {u[Rr].re, T(0)};
+              Z[zidx<M>(N / 2, jj)] = {u[Rr].im, T(0)};
+            });
+          } else {
+            static_for<M>([&](auto Rr) { Z[zidx<M>(k, 4 * Rr + l)] = u[Rr]; });
+          }
         }
       }
       __syncthreads();
