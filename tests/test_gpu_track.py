@@ -38,6 +38,39 @@ FP64_PATHS = ["fp64-fft", "fp64-rowseg"]  # quad-FFT k_track; row-shared k_track
 
 
 @pytest.mark.parametrize("path", FP64_PATHS)
+@pytest.mark.parametrize("sigma,kr", [(15, 0.4), (19, 0.6)])
+def test_kspace_filter_each_path(ctx, sigma, kr, path):
+    """-k through each FP64 kernel at boxes 64 and 80 (the row-shared kernel
+    runs its row tasks as one shared-transform form at 64 and with a branch
+    per task kind at 80), sampled pixels against the oracle; then the FP32
+    row-shared kernel against the FP64 result."""
+    rng = np.random.default_rng(11 + sigma)
+    n = 2 * O.box_nsize(sigma, 4096, 4096)
+    a, b = smooth_pair(rng, int(1.6 * n) + 2, int(1.6 * n) + 7)
+    ctx.set_path("fp64", n, path)
+    try:
+        g = ctx.track(a, b, 1.0, 1.0, sigma, kr=kr)
+        assert ctx.last_path() == path
+    finally:
+        ctx.set_path("fp64", n, None)
+    nx, ny = a.shape
+    pix = rng.choice(nx * ny, 150, replace=False)
+    o = O.flct_pixels(a, b, 1.0, 1.0, sigma, pix, kr=kr)
+    assert_parity((g[0].ravel()[pix], g[1].ravel()[pix], g[2].ravel()[pix]), o,
+                  where=f"-k {kr} N {n} {path}")
+    if path == "fp64-rowseg":
+        ctx.set_path("fp32", n, "fp32-rowseg")
+        try:
+            f = ctx.track(a, b, 1.0, 1.0, sigma, kr=kr, precision="fp32")
+            assert ctx.last_path() == "fp32-rowseg"
+        finally:
+            ctx.set_path("fp32", n, None)
+        assert np.array_equal(f[2], g[2])
+        e = np.hypot(f[0] - g[0], f[1] - g[1])
+        assert np.sqrt(np.mean(e ** 2)) <= 1e-4 and e.max() <= 5e-3, (np.sqrt(np.mean(e ** 2)), e.max())
+
+
+@pytest.mark.parametrize("path", FP64_PATHS)
 @pytest.mark.parametrize("sigma", SIGMAS_FULL)
 def test_box_sizes_full_image(ctx, sigma, path):
     rng = np.random.default_rng(100 + sigma)
